@@ -1,0 +1,189 @@
+"""Benchmark of the MI355X C2C transform path (one JSON line on stdout).
+
+Default workload: 256^3 complex64 forward then unnormalised inverse 3D
+transform (one step = both), inputs resident in HBM, through the library's
+device-pointer entry point on the current torch stream.
+
+value  = whole-job GFLOP/s at 5 N log2 N per transform (10^9 convention; the
+         reference's perf_measures divides by 2^30, examples/helper.cpp:237).
+roofline = the dominant kernel: algorithmic bytes of one launch (16 B per point:
+         one read and one write of the complex64 array it transforms) over its
+         mean launch time from HIP events recorded inside the timed region, as a
+         fraction of MI355X's 8 TB/s.  transform_roofline gives the same
+         fraction for the whole transform (16 B per point per transform).
+cpu_baseline = the FP64 oracle (a scalar C port, 1 thread) on one forward
+         transform of the same workload, timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "fft3d-fpga_amd")
+for _p in (ROOT, PKG_DIR):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK = 8.0e12
+METRIC = "3D C2C GFLOP/s (5N·log2N) + HBM GB/s at 256³ fp32; batched vs plain, 1 GPU"
+
+WORKLOADS = {
+    # name: (dims, batch, seed, round_trip)
+    "3d_256_fwd_inv": ([256, 256, 256], 1, 1, True),
+    "3d_64_fwd": ([64, 64, 64], 1, 0, False),
+    "2d_4096_fwd": ([4096, 4096], 1, 1, False),
+    "1d_4m_fwd": ([1 << 22], 1, 1, False),
+    "3d_512_fwd": ([512, 512, 512], 1, 3, False),
+}
+
+
+def flops_per_transform(dims):
+    n = math.prod(dims)
+    return 5.0 * n * math.log2(n)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="3d_256_fwd_inv", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the torch.fft comparison")
+    return ap.parse_args()
+
+
+def cpu_baseline(dims, seed):
+    """The oracle (FP64 C port, one thread) on one forward transform."""
+    from oracle import oracle
+    x = oracle.uniform(math.prod(dims), seed)
+    oracle.fftnd(x, dims)  # warm-up (page-in)
+    ts = []
+    t_end = time.perf_counter() + 20.0
+    while len(ts) < 3 and (not ts or time.perf_counter() < t_end):
+        t0 = time.perf_counter()
+        oracle.fftnd(x, dims)
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[len(ts) // 2]
+    return {"value": round(flops_per_transform(dims) / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{len(ts)} forward {'x'.join(map(str, dims))} transforms, FP64 oracle, "
+                      f"median {t * 1e3:.1f} ms"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not a bench line)"}))
+        sys.exit(2)
+    import torch
+
+    import fftfpga_amd as fl
+    from fftfpga_amd import datagen
+
+    dims, batch, seed, round_trip = WORKLOADS[args.workload]
+    rank = len(dims)
+    npts = math.prod(dims) * batch
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(datagen.uniform(npts, seed)).to(dev).reshape(batch, *dims)
+    X = torch.empty_like(x)
+    y = torch.empty_like(x)
+    lib = fl.load()
+
+    def step():
+        fl.c2c_dev(x, rank, inverse=False, out=X)
+        if round_trip:
+            fl.c2c_dev(X, rank, inverse=True, out=y)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    per_step = 2 if round_trip else 1
+    max_passes = 4
+    if lib.fftamd_profile_begin(args.steps * per_step * max_passes + 16) != 0:
+        raise RuntimeError(fl.last_error())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    buf = __import__("ctypes").create_string_buffer(1 << 16)
+    n = lib.fftamd_profile_end(buf, len(buf))
+    if n < 0:
+        raise RuntimeError(fl.last_error())
+    kernels = json.loads(buf.value.decode())
+
+    ms = (t1 - t0) * 1e3 / args.steps
+    transforms_per_step = per_step * batch
+    flops = flops_per_transform(dims) * transforms_per_step
+    gflops = flops / (ms * 1e-3) / 1e9
+    alg_bytes_step = 16.0 * npts * per_step
+    gbs = alg_bytes_step / (ms * 1e-3) / 1e9
+
+    # dominant kernel
+    dom = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])
+    dom_name, dom_rec = dom
+    dom_avg_ms = dom_rec["total_ms"] / dom_rec["count"]
+    dom_alg = 16.0 * npts
+    achieved = dom_alg / (dom_avg_ms * 1e-3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dom_name)
+    result = {
+        "metric": METRIC,
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": f"synthetic: splitmix64 seed {seed}, re/im i.i.d. U[-1,1), complex64",
+        "config": {"workload": args.workload, "dims": dims, "batch": batch,
+                   "round_trip": round_trip, "placement": "out-of-place, HBM resident",
+                   "inverse": "unnormalised"},
+        "hbm_gbs_algorithmic": round(gbs, 1),
+        "transform_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK / 1e9,
+                               "unit": "GB/s", "frac": round(gbs * 1e9 / HBM_PEAK, 4)},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "avg_launch_us": round(dom_avg_ms * 1e3, 3)},
+        "kernels": {k: {"count": v["count"], "avg_us": round(1e3 * v["total_ms"] / v["count"], 3)}
+                    for k, v in kernels.items()},
+    }
+    if not args.no_compare:
+        # the vendor library on the same device and data, for reference only
+        for _ in range(3):
+            torch.fft.fftn(x, dim=tuple(range(1, rank + 1)))
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            Xr = torch.fft.fftn(x, dim=tuple(range(1, rank + 1)))
+            if round_trip:
+                torch.fft.ifftn(Xr, dim=tuple(range(1, rank + 1)), norm="forward")
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        ms_t = (tb - ta) * 1e3 / args.steps
+        result["torch_fft_rocfft"] = {"ms_per_step": round(ms_t, 5),
+                                      "gflops": round(flops / (ms_t * 1e-3) / 1e9, 2),
+                                      "speedup_vs_torch": round(ms_t / ms, 3)}
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(dims, seed)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,58 @@
+// engine.h -- internal interface between the C ABI (api.cpp, which sees the
+// drop-in float2/double2 structs) and the HIP engine (which sees HIP's own
+// vector types).  Only plain types cross it.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace fftamd {
+
+enum : unsigned {
+  FLAG_BITREV_1D = 1u,   // leave 1D output bit-reversed within each transform
+  FLAG_SPLIT_MFMA = 2u,  // radix-16 stages on FP16 MFMA with hi+lo operands
+};
+
+struct Timing {
+  double h2d_ms = 0, exec_ms = 0, d2h_ms = 0, stage_in_ms = 0, stage_out_ms = 0;
+};
+
+// ---- device / runtime ----------------------------------------------------------
+int runtime_device_count();            // visible HIP devices (0 when none)
+bool runtime_can_map_host(int dev);    // host-mapped (zero-copy) memory support
+int runtime_init(int dev, bool svm);   // 0 on success
+void runtime_final();
+bool runtime_ready();
+bool runtime_svm();
+int runtime_ensure_device();           // lazy init for the device-pointer API
+const char* last_error();
+void set_error(const char* fmt, ...);
+
+// ---- transforms ------------------------------------------------------------------
+// prec: 0 = FP32 (8-byte points), 1 = FP64 (16-byte points)
+// Host pointers, synchronous, timed: H2D, execution, D2H.
+int host_c2c(int rank, const size_t* dims, size_t batch, const void* hin, void* hout, bool inverse,
+             unsigned flags, int prec, Timing* t);
+// Host pointers through host-mapped staging buffers (the *_svm entry points).
+int host_c2c_mapped(int rank, const size_t* dims, size_t batch, const void* hin, void* hout,
+                    bool inverse, unsigned flags, Timing* t);
+// Host pointers, `count` transforms pipelined in chunks across copy and compute streams.
+int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* hin, void* hout,
+                       bool inverse, Timing* t);
+// Device pointers on the caller's stream (nullptr = the library stream); asynchronous.
+int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
+            unsigned flags, int prec, void* stream);
+
+// Host-only plan description (see engine.hip); returns the pass count or -1.
+int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
+                  long long* out, int cap);
+
+// Pass profiling: while on, every pass launch is bracketed by HIP events on
+// its stream; profile_end synchronises and writes a JSON summary
+// {label: {count, total_ms}}.
+int profile_begin(int max_records);
+int profile_end(char* out, int cap);
+
+void plan_cache_clear();
+size_t plan_cache_size();
+
+}  // namespace fftamd

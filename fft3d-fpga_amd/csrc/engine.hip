@@ -1,0 +1,824 @@
+// engine.hip -- HIP runtime layer: device state, twiddle tables, the
+// axis-pass planner and the host/device transform drivers.
+//
+// Replaces the reference's OpenCL plumbing (api/src/fftfpga.c:72-188,
+// api/src/opencl_utils.c) and the per-call kernel/queue/buffer setup of
+// api/src/fft{1d,2d,3d}.c with: one device, one in-order stream, grow-only
+// HBM staging buffers, plans cached per shape, and twiddle tables generated
+// once in FP64 and stored in the transform precision.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "engine.h"
+#include "fft_pass.h"
+#include "fft_schedule.h"
+
+namespace fftamd {
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "fftfpga(hip): %s\n", g_err);
+}
+const char* last_error() { return g_err; }
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return -1;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+enum Buf { B_IN = 0, B_OUT = 1, B_TMP = 2 };
+
+struct PassDesc {
+  int lg = 0, mode = MODE_CONTIG, epi = EPI_NONE;
+  bool transposing = false;  // writes a different index set than it reads
+  long long in_p = 1, out_p = 1, in_c = 0, out_c = 0, in_k = 0, out_k = 0, in_o = 0, out_o = 0;
+  unsigned ncol = 1, K = 1, kbits = 0;
+  unsigned long long nouter = 1;
+  long long M = 0;
+  unsigned cshift = 0;
+  int src = B_IN, dst = B_OUT;
+};
+
+struct Plan {
+  int prec = PREC_F32;
+  std::vector<PassDesc> passes;
+  size_t total = 0;  // points including batch
+  bool final_copy = false;
+  bool needs_tmp = false;
+  void* tmp = nullptr;
+};
+
+using PlanKey = std::tuple<int, std::vector<size_t>, size_t, unsigned, int, bool>;
+
+struct State {
+  std::recursive_mutex mu;
+  bool ready = false;
+  bool svm = false;
+  int dev = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* d_in = nullptr;
+  size_t d_in_cap = 0;
+  void* d_out = nullptr;
+  size_t d_out_cap = 0;
+  void* h_stage = nullptr;  // host-mapped staging (svm variants)
+  size_t h_stage_cap = 0;
+  std::map<PlanKey, Plan> plans;
+  std::map<std::pair<int, int>, void*> tw_stockham;  // (prec, L)
+  std::map<std::pair<int, long long>, std::pair<void*, void*>> tw_fourstep;  // (prec, M)
+};
+
+static State& S() {
+  static State s;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// pass profiling: HIP events around every pass launch, on the launch stream
+// ---------------------------------------------------------------------------
+struct ProfRec {
+  std::string label;
+  hipEvent_t a, b;
+};
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<ProfRec> recs;
+};
+static Prof& P() {
+  static Prof p;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// twiddles: W_M^t = exp(-2 pi i t / M), evaluated in long double with exact
+// integer phase reduction, rounded once to the storage precision.
+// ---------------------------------------------------------------------------
+static void twiddle_ld(long long t, long long M, long double* c, long double* s) {
+  t %= M;
+  if (t < 0) t += M;
+  // exact quarter-turn symmetries keep 1, -1, i, -i exact
+  long long q = (4 * t) / M;
+  if ((4 * t) % M == 0) {
+    static const long double cs[4][2] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
+    *c = cs[q][0];
+    *s = cs[q][1];
+    return;
+  }
+  const long double twopi = 6.283185307179586476925286766559L;
+  long double a = twopi * (long double)t / (long double)M;
+  *c = cosl(a);
+  *s = -sinl(a);
+}
+
+template <typename R>
+static std::vector<R> stockham_table(int L) {
+  std::vector<R> v(2 * (size_t)std::max(1, sched_twsize(L)));
+  const int np = sched_np(L);
+  for (int st = 1; st < np; ++st) {
+    const int rad = sched_radix(L, st), ns = sched_ns(L, st), off = sched_twoff(L, st);
+    for (int k = 0; k < ns; ++k)
+      for (int r = 1; r < rad; ++r) {
+        long double c, s;
+        twiddle_ld((long long)k * r, (long long)ns * rad, &c, &s);
+        size_t i = (size_t)off + (size_t)k * (rad - 1) + (r - 1);
+        v[2 * i] = (R)c;
+        v[2 * i + 1] = (R)s;
+      }
+  }
+  return v;
+}
+
+static int get_stockham(int prec, int L, const void** out) {
+  State& st = S();
+  auto key = std::make_pair(prec, L);
+  auto it = st.tw_stockham.find(key);
+  if (it != st.tw_stockham.end()) {
+    *out = it->second;
+    return 0;
+  }
+  void* d = nullptr;
+  if (prec == PREC_F32) {
+    auto h = stockham_table<float>(L);
+    HIP_TRY(hipMalloc(&d, h.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    auto h = stockham_table<double>(L);
+    HIP_TRY(hipMalloc(&d, h.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  st.tw_stockham[key] = d;
+  *out = d;
+  return 0;
+}
+
+static unsigned fourstep_hb(long long M) { return (unsigned)((ilog2c(M) + 1) / 2); }
+
+template <typename R>
+static void fourstep_tables(long long M, std::vector<R>& lo, std::vector<R>& hi) {
+  const unsigned hb = fourstep_hb(M);
+  const long long nlo = 1LL << hb, nhi = M >> hb;
+  lo.resize(2 * nlo);
+  hi.resize(2 * nhi);
+  for (long long i = 0; i < nlo; ++i) {
+    long double c, s;
+    twiddle_ld(i, M, &c, &s);
+    lo[2 * i] = (R)c;
+    lo[2 * i + 1] = (R)s;
+  }
+  for (long long j = 0; j < nhi; ++j) {
+    long double c, s;
+    twiddle_ld(j << hb, M, &c, &s);
+    hi[2 * j] = (R)c;
+    hi[2 * j + 1] = (R)s;
+  }
+}
+
+static int get_fourstep(int prec, long long M, const void** lo, const void** hi) {
+  State& st = S();
+  auto key = std::make_pair(prec, M);
+  auto it = st.tw_fourstep.find(key);
+  if (it == st.tw_fourstep.end()) {
+    void *dl = nullptr, *dh = nullptr;
+    if (prec == PREC_F32) {
+      std::vector<float> l, h;
+      fourstep_tables(M, l, h);
+      HIP_TRY(hipMalloc(&dl, l.size() * sizeof(float)));
+      HIP_TRY(hipMalloc(&dh, h.size() * sizeof(float)));
+      HIP_TRY(hipMemcpy(dl, l.data(), l.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(dh, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> l, h;
+      fourstep_tables(M, l, h);
+      HIP_TRY(hipMalloc(&dl, l.size() * sizeof(double)));
+      HIP_TRY(hipMalloc(&dh, h.size() * sizeof(double)));
+      HIP_TRY(hipMemcpy(dl, l.data(), l.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(dh, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    it = st.tw_fourstep.emplace(key, std::make_pair(dl, dh)).first;
+  }
+  *lo = it->second.first;
+  *hi = it->second.second;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// planner: one or two passes per axis, last (contiguous) axis first
+// ---------------------------------------------------------------------------
+static bool is_pow2(size_t x) { return x && !(x & (x - 1)); }
+
+static int split_contig(int prec, int lg, int* lg1, int* lg2) {
+  // L = L1 (strided pass) * L2 (contiguous pass)
+  const int mc = ilog2c(lmax_contig(prec)), ms = ilog2c(lmax_strided(prec));
+  int b = std::min(mc, lg - 1);
+  if (lg - b > ms) return -1;
+  *lg2 = b;
+  *lg1 = lg - b;
+  return 0;
+}
+
+static int split_strided(int prec, int lg, int* lg1, int* lg2) {
+  const int ms = ilog2c(lmax_strided(prec));
+  int a = (lg + 1) / 2;
+  if (a > ms || lg - a > ms) return -1;
+  *lg1 = a;
+  *lg2 = lg - a;
+  return 0;
+}
+
+static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec,
+                      bool aliased, Plan& plan) {
+  plan = Plan();
+  plan.prec = prec;
+  size_t tot = 1;
+  for (int d = 0; d < rank; ++d) tot *= dims[d];
+  plan.total = tot * batch;
+  const bool bitrev = (flags & FLAG_BITREV_1D) && rank == 1;
+  std::vector<PassDesc> ps;
+
+  for (int a = rank - 1; a >= 0; --a) {
+    const long long L = (long long)dims[a];
+    if (L == 1) continue;
+    long long s = 1;
+    for (int d = a + 1; d < rank; ++d) s *= (long long)dims[d];
+    unsigned long long O = batch;
+    for (int d = 0; d < a; ++d) O *= dims[d];
+    const int lg = ilog2c(L);
+    if (s == 1) {
+      if (L <= lmax_contig(prec)) {
+        if (O >= (1ULL << 32)) {
+          set_error("too many lines (%llu) on the contiguous axis", O);
+          return -1;
+        }
+        PassDesc p;
+        p.lg = lg;
+        p.mode = MODE_CONTIG;
+        p.epi = bitrev ? EPI_BITREV : EPI_NONE;
+        p.in_p = p.out_p = 1;
+        p.in_c = p.out_c = L;
+        p.ncol = (unsigned)O;
+        ps.push_back(p);
+      } else {
+        int lg1, lg2;
+        if (split_contig(prec, lg, &lg1, &lg2)) {
+          set_error("contiguous axis of 2^%d points exceeds the supported length", lg);
+          return -1;
+        }
+        const long long L1 = 1LL << lg1, L2 = 1LL << lg2;
+        PassDesc A;  // strided DFT over n1, post-twiddle W_L^{n2 k1}, in place
+        A.lg = lg1;
+        A.mode = MODE_STRIDED;
+        A.epi = EPI_TWIDDLE;
+        A.in_p = A.out_p = L2;
+        A.in_c = A.out_c = 1;
+        A.ncol = (unsigned)L2;
+        A.nouter = O;
+        A.in_o = A.out_o = L;
+        A.M = L;
+        A.cshift = 0;
+        ps.push_back(A);
+        PassDesc B;  // contiguous DFT over n2 for every k1
+        B.lg = lg2;
+        B.mode = MODE_CONTIG;
+        B.transposing = true;
+        B.in_p = 1;
+        B.in_c = L2;
+        B.ncol = (unsigned)L1;
+        B.nouter = O;
+        B.in_o = B.out_o = L;
+        if (bitrev) {  // out[bitrev(k1)*L2 + bitrev(k2)] = X[k1 + L1 k2]
+          B.epi = EPI_BITREV;
+          B.out_p = 1;
+          B.out_c = L2;
+          B.K = 1;
+          // column bit reversal is expressed through the middle index: one column per K
+          B.ncol = 1;
+          B.K = (unsigned)L1;
+          B.in_k = L2;
+          B.out_k = L2;
+          B.in_c = B.out_c = 0;
+          B.kbits = (unsigned)lg1;
+        } else {  // X[k1 + L1 k2] at natural position (strided store)
+          B.epi = EPI_NONE;
+          B.out_p = L1;
+          B.out_c = 1;
+        }
+        ps.push_back(B);
+      }
+    } else {
+      if (L <= lmax_strided(prec)) {
+        PassDesc p;
+        p.lg = lg;
+        p.mode = MODE_STRIDED;
+        p.in_p = p.out_p = s;
+        p.in_c = p.out_c = 1;
+        if (s >= (1LL << 32)) {
+          set_error("axis stride too large");
+          return -1;
+        }
+        p.ncol = (unsigned)s;
+        p.nouter = O;
+        p.in_o = p.out_o = L * s;
+        ps.push_back(p);
+      } else {
+        int lg1, lg2;
+        if (split_strided(prec, lg, &lg1, &lg2)) {
+          set_error("strided axis of 2^%d points exceeds the supported length", lg);
+          return -1;
+        }
+        const long long L1 = 1LL << lg1, L2 = 1LL << lg2;
+        if (L2 * s >= (1LL << 32)) {
+          set_error("axis stride too large");
+          return -1;
+        }
+        PassDesc A;
+        A.lg = lg1;
+        A.mode = MODE_STRIDED;
+        A.epi = EPI_TWIDDLE;
+        A.in_p = A.out_p = L2 * s;
+        A.in_c = A.out_c = 1;
+        A.ncol = (unsigned)(L2 * s);
+        A.nouter = O;
+        A.in_o = A.out_o = L * s;
+        A.M = L;
+        A.cshift = (unsigned)ilog2c(s);
+        ps.push_back(A);
+        PassDesc B;
+        B.lg = lg2;
+        B.mode = MODE_STRIDED;
+        B.transposing = true;
+        B.in_p = s;
+        B.out_p = L1 * s;
+        B.in_c = B.out_c = 1;
+        B.ncol = (unsigned)s;
+        B.K = (unsigned)L1;
+        B.in_k = L2 * s;
+        B.out_k = s;
+        B.nouter = O;
+        B.in_o = B.out_o = L * s;
+        ps.push_back(B);
+      }
+    }
+  }
+
+  // buffer roles: the input is read-only unless in and out alias
+  auto simulate = [&](int first_dst) {
+    int cur = aliased ? B_OUT : B_IN;
+    for (size_t i = 0; i < ps.size(); ++i) {
+      int dst;
+      if (i == 0) dst = first_dst;
+      else if (ps[i].transposing) dst = cur == B_OUT ? B_TMP : B_OUT;
+      else dst = cur;
+      ps[i].src = cur;
+      ps[i].dst = dst;
+      cur = dst;
+    }
+    return cur;
+  };
+  plan.final_copy = false;
+  if (!ps.empty()) {
+    bool done = false;
+    // first pass may write OUT unless it is transposing while aliased
+    const bool first_to_out_ok = !(aliased && ps[0].transposing);
+    if (first_to_out_ok && simulate(B_OUT) == B_OUT) done = true;
+    if (!done && simulate(B_TMP) == B_OUT) done = true;
+    if (!done) {
+      simulate(first_to_out_ok ? B_OUT : B_TMP);
+      plan.final_copy = true;
+    }
+  } else {
+    plan.final_copy = !aliased;  // length-1 transform: identity copy
+  }
+  plan.passes = ps;
+  plan.needs_tmp = false;
+  for (auto& p : plan.passes)
+    if (p.src == B_TMP || p.dst == B_TMP) plan.needs_tmp = true;
+  return 0;
+}
+
+static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
+                    Plan** out) {
+  State& st = S();
+  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & FLAG_BITREV_1D, prec,
+              aliased);
+  auto it = st.plans.find(key);
+  if (it == st.plans.end()) {
+    Plan p;
+    if (build_plan(rank, dims, batch, flags, prec, aliased, p)) return -1;
+    if (p.needs_tmp) {
+      const size_t bytes = p.total * (prec == PREC_F32 ? 8 : 16);
+      HIP_TRY(hipMalloc(&p.tmp, bytes));
+    }
+    it = st.plans.emplace(key, std::move(p)).first;
+  }
+  *out = &it->second;
+  return 0;
+}
+
+static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipStream_t stream) {
+  const size_t esz = plan.prec == PREC_F32 ? 8 : 16;
+  auto ptr = [&](int b) -> void* {
+    return b == B_IN ? const_cast<void*>(din) : (b == B_OUT ? dout : plan.tmp);
+  };
+  int cur = B_IN;
+  for (auto& p : plan.passes) {
+    PassArgs a{};
+    a.in = ptr(p.src);
+    a.out = ptr(p.dst);
+    a.in_p = p.in_p;
+    a.out_p = p.out_p;
+    a.in_c = p.in_c;
+    a.out_c = p.out_c;
+    a.in_k = p.in_k;
+    a.out_k = p.out_k;
+    a.in_o = p.in_o;
+    a.out_o = p.out_o;
+    a.ncol = p.ncol;
+    const int L = 1 << p.lg;
+    const int C = geom_cols(L, p.mode);
+    a.ncb = (p.ncol + C - 1) / C;
+    a.K = p.K;
+    a.kbits = p.kbits;
+    a.inverse = inverse ? 1 : 0;
+    if (sched_np(L) > 1) {
+      if (get_stockham(plan.prec, L, &a.tw)) return -1;
+    }
+    if (p.epi == EPI_TWIDDLE) {
+      if (get_fourstep(plan.prec, p.M, &a.ptw_lo, &a.ptw_hi)) return -1;
+      a.ptw_hb = fourstep_hb(p.M);
+      a.ptw_mmask = (unsigned)(p.M - 1);
+      a.ptw_cshift = p.cshift;
+    }
+    const unsigned long long per_outer = (unsigned long long)a.ncb * a.K;
+    const unsigned long long max_blocks = (0xFFFFFFFFULL / (unsigned)geom_threads(L, p.mode));
+    const unsigned long long outer_per_launch = std::max(1ULL, max_blocks / per_outer);
+    for (unsigned long long o0 = 0; o0 < p.nouter; o0 += outer_per_launch) {
+      const unsigned long long no = std::min(outer_per_launch, p.nouter - o0);
+      a.outer0 = o0;
+      Prof& pf = P();
+      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
+      int rc = launch_fft_pass(plan.prec, p.lg, p.mode, p.epi, a, per_outer * no, stream);
+      if (prof) {
+        HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
+        char lab[96];
+        snprintf(lab, sizeof(lab), "fft_pass_%s_L%d%s%s", plan.prec == PREC_F32 ? "f32" : "f64", L,
+                 p.mode == MODE_CONTIG ? "_contig" : "_strided",
+                 p.epi == EPI_TWIDDLE ? "_twiddle" : (p.epi == EPI_BITREV ? "_bitrev" : ""));
+        pf.recs.push_back({lab, pf.pool[pf.used], pf.pool[pf.used + 1]});
+        pf.used += 2;
+      }
+      if (rc) {
+        set_error("pass launch failed (lg=%d mode=%d epi=%d rc=%d): %s", p.lg, p.mode, p.epi, rc,
+                  hipGetErrorString(hipGetLastError()));
+        return -1;
+      }
+    }
+    cur = p.dst;
+  }
+  if (plan.final_copy) {
+    const void* src = plan.passes.empty() ? din : ptr(cur);
+    if (src != dout) HIP_TRY(hipMemcpyAsync(dout, src, plan.total * esz, hipMemcpyDeviceToDevice, stream));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// runtime
+// ---------------------------------------------------------------------------
+int runtime_device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+bool runtime_can_map_host(int dev) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeCanMapHostMemory, dev) != hipSuccess) return false;
+  return v != 0;
+}
+
+static void release_locked() {
+  State& st = S();
+  for (auto& kv : st.plans)
+    if (kv.second.tmp) (void)hipFree(kv.second.tmp);
+  st.plans.clear();
+  for (auto& kv : st.tw_stockham) (void)hipFree(kv.second);
+  st.tw_stockham.clear();
+  for (auto& kv : st.tw_fourstep) {
+    (void)hipFree(kv.second.first);
+    (void)hipFree(kv.second.second);
+  }
+  st.tw_fourstep.clear();
+  if (st.d_in) (void)hipFree(st.d_in);
+  if (st.d_out) (void)hipFree(st.d_out);
+  if (st.h_stage) (void)hipHostFree(st.h_stage);
+  st.d_in = st.d_out = st.h_stage = nullptr;
+  st.d_in_cap = st.d_out_cap = st.h_stage_cap = 0;
+  for (auto& e : st.ev)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+  if (st.stream) (void)hipStreamDestroy(st.stream);
+  st.stream = nullptr;
+  st.ready = false;
+  st.svm = false;
+}
+
+int runtime_init(int dev, bool svm) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (st.ready) release_locked();
+  HIP_TRY(hipSetDevice(dev));
+  HIP_TRY(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+  for (auto& e : st.ev) HIP_TRY(hipEventCreate(&e));
+  st.dev = dev;
+  st.svm = svm;
+  st.ready = true;
+  return 0;
+}
+
+void runtime_final() {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (st.ready) {
+    (void)hipSetDevice(st.dev);
+    (void)hipStreamSynchronize(st.stream);
+  }
+  release_locked();
+}
+
+bool runtime_ready() { return S().ready; }
+bool runtime_svm() { return S().ready && S().svm; }
+
+int runtime_ensure_device() {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (st.ready) return 0;
+  if (runtime_device_count() <= 0) {
+    set_error("no HIP device visible");
+    return -1;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  return runtime_init(dev, false);
+}
+
+void plan_cache_clear() {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (st.ready) (void)hipDeviceSynchronize();
+  for (auto& kv : st.plans)
+    if (kv.second.tmp) (void)hipFree(kv.second.tmp);
+  st.plans.clear();
+}
+
+size_t plan_cache_size() { return S().plans.size(); }
+
+int profile_begin(int max_records) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (runtime_ensure_device()) return -1;
+  Prof& pf = P();
+  const size_t need = 2 * (size_t)std::max(1, max_records);
+  while (pf.pool.size() < need) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    pf.pool.push_back(e);
+  }
+  pf.used = 0;
+  pf.recs.clear();
+  pf.on = true;
+  return 0;
+}
+
+int profile_end(char* out, int cap) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  Prof& pf = P();
+  pf.on = false;
+  std::map<std::string, std::pair<long long, double>> agg;
+  for (auto& r : pf.recs) {
+    HIP_TRY(hipEventSynchronize(r.b));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+    auto& e = agg[r.label];
+    e.first += 1;
+    e.second += ms;
+  }
+  std::string js = "{";
+  bool first = true;
+  for (auto& kv : agg) {
+    char buf[192];
+    snprintf(buf, sizeof(buf), "%s\"%s\": {\"count\": %lld, \"total_ms\": %.6f}", first ? "" : ", ",
+             kv.first.c_str(), kv.second.first, kv.second.second);
+    js += buf;
+    first = false;
+  }
+  js += "}";
+  pf.recs.clear();
+  pf.used = 0;
+  if (out && cap > 0) {
+    snprintf(out, (size_t)cap, "%s", js.c_str());
+  }
+  return (int)js.size();
+}
+
+static bool check_dims(int rank, const size_t* dims, size_t batch);
+
+// Host-only description of a plan (no device needed): 22 int64 fields per
+// pass, then one final-copy flag.
+int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
+                  long long* out, int cap) {
+  if (!check_dims(rank, dims, batch)) return -1;
+  Plan p;
+  if (build_plan(rank, dims, batch, flags, prec, aliased, p)) return -1;
+  const int NF = 22;
+  const int n = (int)p.passes.size();
+  if (out == nullptr) return n;
+  if (cap < NF * n + 1) {
+    set_error("describe_plan: buffer too small");
+    return -1;
+  }
+  for (int i = 0; i < n; ++i) {
+    const PassDesc& d = p.passes[i];
+    long long* f = out + NF * i;
+    long long v[NF] = {d.lg, d.mode, d.epi, d.transposing, d.in_p, d.out_p, d.in_c, d.out_c,
+                       d.in_k, d.out_k, d.in_o, d.out_o, d.ncol, d.K, d.kbits,
+                       (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
+                       geom_cols(1 << d.lg, d.mode), geom_threads(1 << d.lg, d.mode)};
+    for (int k = 0; k < NF; ++k) f[k] = v[k];
+  }
+  out[NF * n] = p.final_copy ? 1 : 0;
+  return n;
+}
+
+static int ensure_buf(void** p, size_t* cap, size_t bytes) {
+  if (*cap >= bytes) return 0;
+  if (*p) HIP_TRY(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  HIP_TRY(hipMalloc(p, bytes));
+  *cap = bytes;
+  return 0;
+}
+
+static bool check_dims(int rank, const size_t* dims, size_t batch) {
+  if (rank < 1 || rank > 3 || batch == 0) {
+    set_error("bad rank/batch");
+    return false;
+  }
+  for (int d = 0; d < rank; ++d)
+    if (!is_pow2(dims[d])) {
+      set_error("dimension %zu is not a power of two", dims[d]);
+      return false;
+    }
+  return true;
+}
+
+static double ms_between(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+  return (double)ms;
+}
+
+int host_c2c(int rank, const size_t* dims, size_t batch, const void* hin, void* hout, bool inverse,
+             unsigned flags, int prec, Timing* t) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (!st.ready) {
+    set_error("library not initialised (call fpga_initialize first)");
+    return -1;
+  }
+  if (!check_dims(rank, dims, batch)) return -1;
+  HIP_TRY(hipSetDevice(st.dev));
+  Plan* plan = nullptr;
+  if (get_plan(rank, dims, batch, flags, prec, false, &plan)) return -1;
+  const size_t bytes = plan->total * (prec == PREC_F32 ? 8 : 16);
+  if (ensure_buf(&st.d_in, &st.d_in_cap, bytes) || ensure_buf(&st.d_out, &st.d_out_cap, bytes))
+    return -1;
+  HIP_TRY(hipEventRecord(st.ev[0], st.stream));
+  HIP_TRY(hipMemcpyAsync(st.d_in, hin, bytes, hipMemcpyHostToDevice, st.stream));
+  HIP_TRY(hipEventRecord(st.ev[1], st.stream));
+  if (run_plan(*plan, st.d_in, st.d_out, inverse, st.stream)) return -1;
+  HIP_TRY(hipEventRecord(st.ev[2], st.stream));
+  HIP_TRY(hipMemcpyAsync(hout, st.d_out, bytes, hipMemcpyDeviceToHost, st.stream));
+  HIP_TRY(hipEventRecord(st.ev[3], st.stream));
+  HIP_TRY(hipStreamSynchronize(st.stream));
+  if (t) {
+    t->h2d_ms = ms_between(st.ev[0], st.ev[1]);
+    t->exec_ms = ms_between(st.ev[1], st.ev[2]);
+    t->d2h_ms = ms_between(st.ev[2], st.ev[3]);
+  }
+  return 0;
+}
+
+int host_c2c_mapped(int rank, const size_t* dims, size_t batch, const void* hin, void* hout,
+                    bool inverse, unsigned flags, Timing* t) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (!st.ready) {
+    set_error("library not initialised (call fpga_initialize first)");
+    return -1;
+  }
+  if (!check_dims(rank, dims, batch)) return -1;
+  HIP_TRY(hipSetDevice(st.dev));
+  Plan* plan = nullptr;
+  if (get_plan(rank, dims, batch, flags, PREC_F32, false, &plan)) return -1;
+  const size_t bytes = plan->total * 8;
+  if (ensure_buf(&st.d_in, &st.d_in_cap, bytes) || ensure_buf(&st.d_out, &st.d_out_cap, bytes))
+    return -1;
+  if (st.h_stage_cap < bytes) {
+    if (st.h_stage) HIP_TRY(hipHostFree(st.h_stage));
+    st.h_stage = nullptr;
+    st.h_stage_cap = 0;
+    HIP_TRY(hipHostMalloc(&st.h_stage, bytes, hipHostMallocMapped));
+    st.h_stage_cap = bytes;
+  }
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  std::memcpy(st.h_stage, hin, bytes);
+  auto t1 = clk::now();
+  HIP_TRY(hipEventRecord(st.ev[0], st.stream));
+  HIP_TRY(hipMemcpyAsync(st.d_in, st.h_stage, bytes, hipMemcpyHostToDevice, st.stream));
+  HIP_TRY(hipEventRecord(st.ev[1], st.stream));
+  if (run_plan(*plan, st.d_in, st.d_out, inverse, st.stream)) return -1;
+  HIP_TRY(hipEventRecord(st.ev[2], st.stream));
+  HIP_TRY(hipMemcpyAsync(st.h_stage, st.d_out, bytes, hipMemcpyDeviceToHost, st.stream));
+  HIP_TRY(hipEventRecord(st.ev[3], st.stream));
+  HIP_TRY(hipStreamSynchronize(st.stream));
+  auto t2 = clk::now();
+  std::memcpy(hout, st.h_stage, bytes);
+  auto t3 = clk::now();
+  if (t) {
+    t->stage_in_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    t->h2d_ms = ms_between(st.ev[0], st.ev[1]);
+    t->exec_ms = ms_between(st.ev[1], st.ev[2]);
+    t->d2h_ms = ms_between(st.ev[2], st.ev[3]);
+    t->stage_out_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+  }
+  return 0;
+}
+
+int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* hin, void* hout,
+                       bool inverse, Timing* t) {
+  // Single-chunk form: one copy in, one batched execution, one copy out; the
+  // wall-clock time of the whole call is reported as exec_ms.
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  Timing tt;
+  int rc = host_c2c(rank, dims, count, hin, hout, inverse, 0u, PREC_F32, &tt);
+  auto t1 = clk::now();
+  if (rc) return rc;
+  if (t) {
+    *t = Timing();
+    t->exec_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+  return 0;
+}
+
+int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
+            unsigned flags, int prec, void* stream) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (runtime_ensure_device()) return -1;
+  if (!check_dims(rank, dims, batch)) return -1;
+  if (!din || !dout) {
+    set_error("null device pointer");
+    return -1;
+  }
+  Plan* plan = nullptr;
+  const bool aliased = din == dout;
+  if (get_plan(rank, dims, batch, flags, prec, aliased, &plan)) return -1;
+  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : st.stream;
+  return run_plan(*plan, din, dout, inverse, s);
+}
+
+}  // namespace fftamd

@@ -1,0 +1,57 @@
+// fft_schedule.h -- compile-time radix schedule and launch geometry of the
+// Stockham line-FFT pass kernels.  Shared by the device kernels and by the
+// host code that builds twiddle tables and launch grids, so both always agree.
+//
+// A "line" is one 1D transform of L = 2^lg points.  A pass kernel holds C
+// lines per workgroup, PPT points of each line per thread (TPL threads per
+// line), and runs NP radix stages: the first of radix 2^(lg - 4(NP-1)), the
+// rest radix 16.  Stage s sees sub-transforms of span NS(s) = prod of the
+// radices before it (Stockham autosort: natural order in, natural order out).
+#pragma once
+
+namespace fftamd {
+
+enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1 };
+// EPI_TWIDDLE multiplies point k of column c by W_M^{(c >> cshift) * k}
+// (four-step split of a long axis); EPI_BITREV stores point k at bitrev(k).
+enum PassEpilogue { EPI_NONE = 0, EPI_TWIDDLE = 1, EPI_BITREV = 2 };
+enum Precision { PREC_F32 = 0, PREC_F64 = 1 };
+
+constexpr int ilog2c(long long x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
+
+// ---- radix schedule ------------------------------------------------------
+constexpr int sched_np(int L) { return L <= 16 ? 1 : (ilog2c(L) + 3) / 4; }
+constexpr int sched_radix(int L, int s) {
+  return s == 0 ? (1 << (ilog2c(L) - 4 * (sched_np(L) - 1))) : 16;
+}
+constexpr int sched_ns(int L, int s) {
+  int n = 1;
+  for (int i = 0; i < s; ++i) n *= sched_radix(L, i);
+  return n;
+}
+// Twiddles of stage s >= 1 sit at [twoff(s) + k*(R-1) + (r-1)], k < NS(s), 1 <= r < R,
+// holding W_{NS*R}^{k*r}.
+constexpr int sched_twoff(int L, int s) {
+  int o = 0;
+  for (int i = 1; i < s; ++i) o += sched_ns(L, i) * (sched_radix(L, i) - 1);
+  return o;
+}
+constexpr int sched_twsize(int L) { return sched_twoff(L, sched_np(L)); }
+
+// ---- geometry -------------------------------------------------------------
+constexpr int geom_ppt(int L) { return L < 16 ? L : 16; }
+constexpr int geom_tpl(int L) { return L / geom_ppt(L); }
+// Lines per workgroup.  Contiguous lines: ~256 threads per workgroup.
+// Strided lines: enough adjacent columns for >= 128-byte row segments.
+constexpr int geom_cols(int L, int mode) {
+  return mode == MODE_CONTIG ? (geom_tpl(L) >= 256 ? 1 : 256 / geom_tpl(L))
+                             : (L >= 256 ? 16 : (L == 128 ? 32 : 64));
+}
+constexpr int geom_threads(int L, int mode) { return geom_cols(L, mode) * geom_tpl(L); }
+
+// Largest line held by one workgroup (LDS budget).  Longer axes are split
+// four-step into two passes.
+constexpr int lmax_contig(int prec) { return prec == PREC_F32 ? 8192 : 4096; }
+constexpr int lmax_strided(int prec) { return prec == PREC_F32 ? 512 : 256; }
+
+}  // namespace fftamd

@@ -1,0 +1,74 @@
+/*
+ * fftfpga_hip.h -- ABI-additive extensions of the FFTFPGA C API for callers
+ * that already hold data in GPU memory (PyTorch tensors, HIP applications).
+ *
+ * The reference has no counterpart for these entry points: it only accepts
+ * host pointers and copies over PCIe on every call (api/src/fft3d.c:242-264).
+ * They run the same gfx950 kernels as fftfpga.h, asynchronously on the
+ * caller's HIP stream (a hipStream_t passed as void*; NULL selects the
+ * library's own stream).  All return 0 on success and a negative code on
+ * error, with the reason in fftamd_last_error().
+ *
+ * Layout conventions are those of fftfpga.h: interleaved complex, row-major,
+ * last dimension contiguous, transforms of a batch contiguous.
+ * d_in == d_out (in place) is allowed.
+ */
+#ifndef FFTFPGA_HIP_H
+#define FFTFPGA_HIP_H
+
+#include "fftfpga.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* flags */
+#define FFTAMD_BITREV_1D 0x1u  /* rank 1: leave each transform bit-reversed, as fftfpgaf_c2c_1d */
+
+/* Single precision, rank 1..3, every dimension a power of two, `batch`
+ * contiguous transforms.  inverse != 0: unnormalised backward transform. */
+int fftamd_c2c_dev(int rank, const unsigned *dims, unsigned batch, const float2 *d_in, float2 *d_out,
+                   int inverse, unsigned flags, void *stream);
+
+/* Double precision twin of fftamd_c2c_dev. */
+int fftamd_z2z_dev(int rank, const unsigned *dims, unsigned batch, const double2 *d_in,
+                   double2 *d_out, int inverse, unsigned flags, void *stream);
+
+/* Device-pointer twin of fftfpgaf_c2c_3d_ddr / _3d_ddr_batch: how_many
+ * contiguous N^3 transforms (how_many >= 1). */
+int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2 *d_in, float2 *d_out, bool inv,
+                            unsigned how_many, void *stream);
+
+/* Last error message of the calling thread ("" if none). */
+const char *fftamd_last_error(void);
+
+/* Drop every cached plan (twiddle tables stay until fpga_final). */
+void fftamd_plan_cache_clear(void);
+
+/* Number of cached plans. */
+size_t fftamd_plan_cache_size(void);
+
+/* Per-kernel timing: after fftamd_profile_begin, every kernel launch of the
+ * library is bracketed by HIP events on its own stream (up to max_records
+ * launches).  fftamd_profile_end synchronises, stops recording and writes a
+ * JSON object {"<kernel label>": {"count": n, "total_ms": t}, ...} into
+ * json_out (truncated to cap bytes); returns the full JSON length or -1. */
+int fftamd_profile_begin(int max_records);
+int fftamd_profile_end(char *json_out, int cap);
+
+/* Describe, without touching the device, the passes the library would run
+ * for a transform: 22 int64 fields per pass (line log2 length, mode,
+ * epilogue, transposing, point/column/middle/outer strides in and out,
+ * column count, middle extent, middle bit-reversal bits, outer extent,
+ * four-step modulus, column shift, source and destination buffer, columns
+ * per workgroup, threads per workgroup) followed by one final-copy flag.
+ * prec: 0 FP32, 1 FP64.  With out == NULL returns the pass count; otherwise
+ * writes at most cap values and returns the pass count, or -1 on error. */
+int fftamd_describe_plan(int rank, const unsigned *dims, unsigned batch, unsigned flags, int prec,
+                         int aliased, long long *out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFTFPGA_HIP_H */

@@ -1,0 +1,200 @@
+"""CPU emulation of the GPU pass structure, before and beside the GPU runs.
+
+1. Stockham stages: the per-thread index maps of k_fft_pass (stage loads at
+   j + r*L/R, twiddle W_{NS R}^{(j mod NS) r}, stores at
+   (j/NS)*NS*R + (j mod NS) + r*NS, radix schedule of fft_schedule.h) are
+   replayed in numpy for every supported line length and must give the DFT.
+2. Pass plans: the library's planner (fftamd_describe_plan, host-only) is
+   replayed pass by pass with the kernel's address formulas; every read and
+   write must stay inside the buffer, each pass must write every output
+   element exactly once, and the composition must equal the oracle's
+   transform (four-step twiddles, bit-reversed 1D order and buffer roles
+   included).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+NF = 22
+
+
+def sched(L):
+    lg = L.bit_length() - 1
+    np_ = 1 if L <= 16 else (lg + 3) // 4
+    rad = [(1 << (lg - 4 * (np_ - 1)))] + [16] * (np_ - 1)
+    ns = [int(np.prod(rad[:s])) for s in range(np_)]
+    return np_, rad, ns
+
+
+def emulate_line(x):
+    """Replays one line through the kernel's stage index maps (forward)."""
+    L = x.size
+    if L == 1:
+        return x.copy()
+    NP, RAD, NS = sched(L)
+    PPT = min(L, 16)
+    TPL = L // PPT
+    v = np.zeros((TPL, PPT), np.complex128)
+    lds = np.full(L, np.nan + 0j)
+    for S in range(NP):
+        R, ns = RAD[S], NS[S]
+        nb = PPT // R
+        for tau in range(TPL):
+            for b in range(nb):
+                j = tau + b * TPL
+                for r in range(R):
+                    v[tau, b * R + r] = x[j + r * (L // R)] if S == 0 else lds[j + r * (L // R)]
+                if S > 0:
+                    k = j & (ns - 1)
+                    for r in range(1, R):
+                        v[tau, b * R + r] *= np.exp(-2j * np.pi * k * r / (ns * R))
+                v[tau, b * R:(b + 1) * R] = np.fft.fft(v[tau, b * R:(b + 1) * R])
+        if S + 1 < NP:
+            new = np.full(L, np.nan + 0j)
+            for tau in range(TPL):
+                for b in range(nb):
+                    j = tau + b * TPL
+                    base = (j // ns) * ns * R + (j & (ns - 1))
+                    for r in range(R):
+                        assert np.isnan(new[base + r * ns])  # written once
+                        new[base + r * ns] = v[tau, b * R + r]
+            lds = new
+    R, ns = RAD[-1], NS[-1]
+    X = np.full(L, np.nan + 0j)
+    for tau in range(TPL):
+        for b in range(PPT // R):
+            j = tau + b * TPL
+            for r in range(R):
+                X[j + r * ns] = v[tau, b * R + r]
+    return X
+
+
+@pytest.mark.parametrize("lg", range(0, 14))
+def test_stockham_stage_maps(lg):
+    L = 1 << lg
+    rng = np.random.default_rng(lg)
+    x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
+    X = emulate_line(x)
+    assert not np.isnan(X).any()
+    assert np.allclose(X, np.fft.fft(x), rtol=1e-12, atol=1e-10)
+
+
+def describe(fl, dims, batch, flags=0, prec=0, aliased=False):
+    lib = fl.load()
+    rank = len(dims)
+    cd = (ctypes.c_uint * rank)(*dims)
+    n = lib.fftamd_describe_plan(rank, cd, batch, flags, prec, int(aliased), None, 0)
+    assert n >= 0, fl.last_error()
+    buf = (ctypes.c_longlong * (NF * n + 1))()
+    assert lib.fftamd_describe_plan(rank, cd, batch, flags, prec, int(aliased), buf, len(buf)) == n
+    arr = np.frombuffer(buf, dtype=np.int64)
+    keys = ["lg", "mode", "epi", "transposing", "in_p", "out_p", "in_c", "out_c", "in_k", "out_k",
+            "in_o", "out_o", "ncol", "K", "kbits", "nouter", "M", "cshift", "src", "dst", "C", "NT"]
+    return [dict(zip(keys, (int(v) for v in arr[NF * i:NF * (i + 1)]))) for i in range(n)], \
+        bool(arr[NF * n])
+
+
+def brev(x, bits):
+    r = np.zeros_like(x)
+    for i in range(bits):
+        r |= ((x >> i) & 1) << (bits - 1 - i)
+    return r
+
+
+def run_plan(passes, final_copy, x, inverse, aliased=False):
+    total = x.size
+    bufs = {0: x.astype(np.complex128).copy(), 1: np.zeros(total, np.complex128),
+            2: np.zeros(total, np.complex128)}
+    if aliased:
+        bufs[1] = bufs[0]
+    cur = 1 if aliased else 0
+    for p in passes:
+        L = 1 << p["lg"]
+        C = p["C"]
+        ncb = (p["ncol"] + C - 1) // C
+        # every active line (o, k, col); the grid covers col < ncb*C, inactive ones store nothing
+        o, k, col = np.meshgrid(np.arange(p["nouter"]), np.arange(p["K"]), np.arange(ncb * C),
+                                indexing="ij")
+        o, k, col = o.reshape(-1), k.reshape(-1), col.reshape(-1)
+        act = col < p["ncol"]
+        o, k, col = o[act], k[act], col[act]
+        ko = brev(k, p["kbits"]) if p["kbits"] else k
+        pts = np.arange(L)
+        iidx = (o * p["in_o"] + k * p["in_k"] + col * p["in_c"])[:, None] + pts[None, :] * p["in_p"]
+        oidx = (o * p["out_o"] + ko * p["out_k"] + col * p["out_c"])[:, None] + pts[None, :] * p["out_p"]
+        assert iidx.min() >= 0 and iidx.max() < total
+        assert oidx.min() >= 0 and oidx.max() < total
+        assert np.unique(oidx).size == oidx.size == total, "each output written exactly once"
+        assert p["src"] == cur
+        src = bufs[p["src"]]
+        lines = src[iidx]
+        if inverse:
+            X = np.fft.ifft(lines, axis=1) * L
+        else:
+            X = np.fft.fft(lines, axis=1)
+        if p["epi"] == 1:  # four-step post-twiddle, applied in the forward (swapped) domain
+            t = ((col >> p["cshift"])[:, None] * pts[None, :]) % p["M"]
+            w = np.exp(-2j * np.pi * t / p["M"])
+            X = np.conj(w) * X if inverse else w * X
+        if p["epi"] == 2:  # point k stored at bitrev(k)
+            X = X[:, brev(pts, p["lg"])]
+        if p["src"] == p["dst"]:
+            assert not p["transposing"]
+            assert np.array_equal(np.sort(iidx.reshape(-1)), np.sort(oidx.reshape(-1)))
+        dst = bufs[p["dst"]]
+        dst[oidx] = X
+        cur = p["dst"]
+    if final_copy:
+        bufs[1][:] = bufs[cur]
+        cur = 1
+    assert cur == 1
+    return bufs[1]
+
+
+CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16], 3, 0),
+         ([1024, 8], 1, 0), ([2048, 32], 1, 0), ([8, 4096], 1, 0), ([1, 1, 1], 2, 0),
+         ([2, 1, 4], 1, 0), ([256], 5, 1), ([1 << 14], 2, 1), ([1 << 14], 2, 0), ([1 << 16], 1, 1),
+         ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0)]
+
+
+@pytest.mark.parametrize("dims,batch,flags", CASES)
+@pytest.mark.parametrize("inverse", [False, True])
+def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
+    passes, final_copy = describe(fl, dims, batch, flags)
+    n = int(np.prod(dims)) * batch
+    x = orc.uniform(n, 77 + n % 101)
+    got = run_plan(passes, final_copy, x, inverse)
+    if flags & 1:
+        ref = orc.c2c_1d_expected(x, dims[0], batch, inverse=inverse).reshape(-1)
+    else:
+        ref = orc.fftnd(x, dims, batch=batch, inverse=inverse).reshape(-1)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("dims,batch,flags", [([64, 64, 64], 1, 0), ([2048, 32], 1, 0),
+                                              ([1 << 14], 1, 1), ([1 << 14], 1, 0)])
+def test_plan_replay_in_place(fl, orc, dims, batch, flags):
+    passes, final_copy = describe(fl, dims, batch, flags, aliased=True)
+    n = int(np.prod(dims)) * batch
+    x = orc.uniform(n, 5)
+    got = run_plan(passes, final_copy, x, False, aliased=True)
+    ref = (orc.c2c_1d_expected(x, dims[0], batch) if flags & 1 else
+           orc.fftnd(x, dims, batch=batch)).reshape(-1)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_plan_shapes_of_headline_configs(fl):
+    # 256^3: three single passes (x contiguous, y and z strided), no temporary buffer
+    p, fc = describe(fl, [256, 256, 256], 1)
+    assert [q["mode"] for q in p] == [0, 1, 1] and not fc and all(q["dst"] == 1 for q in p)
+    # 1D 2^22 bit-reversed: four-step, strided 512 then contiguous 8192
+    p, fc = describe(fl, [1 << 22], 1, 1)
+    assert [(q["lg"], q["mode"], q["epi"]) for q in p] == [(9, 1, 1), (13, 0, 2)]
+    # 2D 4096^2: contiguous 4096, then the strided axis split 64 x 64
+    p, fc = describe(fl, [4096, 4096], 1)
+    assert [(q["lg"], q["mode"]) for q in p] == [(12, 0), (6, 1), (6, 1)]
+    # every workgroup stays within 1024 threads
+    for dims in ([2, 2, 2], [8192], [512, 512], [1 << 22]):
+        for q in describe(fl, dims, 1)[0]:
+            assert q["NT"] <= 1024
