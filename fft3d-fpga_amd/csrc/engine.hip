@@ -787,16 +787,99 @@ int host_c2c_mapped(int rank, const size_t* dims, size_t batch, const void* hin,
   return 0;
 }
 
+// Pipelined host batch (the GPU counterpart of the reference's 4-buffer
+// rotation, api/src/fft3d.c:551-705): chunks of whole transforms flow
+//   host memcpy -> pinned[slot] -> H2D (s_h2d) -> transform (s_exec) -> D2H (s_d2h)
+//   -> pinned[slot] -> host memcpy
+// with two slots, so the copies of chunk i+1 and i-1 overlap the transform of
+// chunk i.  exec_ms is the wall-clock time of the whole call, as the
+// reference reports it (api/src/fft3d.c:489,861).
+struct Pipe {
+  hipStream_t s[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t h2d[2] = {}, ex[2] = {}, d2h[2] = {};
+  void* pin_in[2] = {nullptr, nullptr};
+  void* pin_out[2] = {nullptr, nullptr};
+  void* d_in[2] = {nullptr, nullptr};
+  void* d_out[2] = {nullptr, nullptr};
+  ~Pipe() {
+    for (auto st : s)
+      if (st) (void)hipStreamSynchronize(st);
+    for (int i = 0; i < 2; ++i) {
+      if (pin_in[i]) (void)hipHostFree(pin_in[i]);
+      if (pin_out[i]) (void)hipHostFree(pin_out[i]);
+      if (d_in[i]) (void)hipFree(d_in[i]);
+      if (d_out[i]) (void)hipFree(d_out[i]);
+      if (h2d[i]) (void)hipEventDestroy(h2d[i]);
+      if (ex[i]) (void)hipEventDestroy(ex[i]);
+      if (d2h[i]) (void)hipEventDestroy(d2h[i]);
+    }
+    for (auto st : s)
+      if (st) (void)hipStreamDestroy(st);
+  }
+};
+
 int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* hin, void* hout,
                        bool inverse, Timing* t) {
-  // Single-chunk form: one copy in, one batched execution, one copy out; the
-  // wall-clock time of the whole call is reported as exec_ms.
   using clk = std::chrono::steady_clock;
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (!st.ready) {
+    set_error("library not initialised (call fpga_initialize first)");
+    return -1;
+  }
+  if (!check_dims(rank, dims, count)) return -1;
+  HIP_TRY(hipSetDevice(st.dev));
   auto t0 = clk::now();
-  Timing tt;
-  int rc = host_c2c(rank, dims, count, hin, hout, inverse, 0u, PREC_F32, &tt);
+  size_t per = 1;
+  for (int d = 0; d < rank; ++d) per *= dims[d];
+  const size_t tbytes = per * 8;
+  // chunks of whole transforms, about 64 MiB each
+  const size_t chunk = std::max<size_t>(1, std::min<size_t>(count, (64u << 20) / std::max<size_t>(tbytes, 1)));
+  const size_t nchunks = (count + chunk - 1) / chunk;
+  const size_t cbytes = chunk * tbytes;
+  Pipe pp;
+  for (auto& s : pp.s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(hipEventCreateWithFlags(&pp.h2d[i], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&pp.ex[i], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&pp.d2h[i], hipEventDisableTiming));
+    HIP_TRY(hipHostMalloc(&pp.pin_in[i], cbytes, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&pp.pin_out[i], cbytes, hipHostMallocDefault));
+    HIP_TRY(hipMalloc(&pp.d_in[i], cbytes));
+    HIP_TRY(hipMalloc(&pp.d_out[i], cbytes));
+  }
+  const char* src = static_cast<const char*>(hin);
+  char* dst = static_cast<char*>(hout);
+  auto nb = [&](size_t i) { return std::min(chunk, count - i * chunk) * tbytes; };
+  auto drain = [&](size_t i) -> int {  // copy chunk i out of its pinned slot
+    const int slot = (int)(i & 1);
+    HIP_TRY(hipEventSynchronize(pp.d2h[slot]));
+    std::memcpy(dst + i * cbytes, pp.pin_out[slot], nb(i));
+    return 0;
+  };
+  for (size_t i = 0; i < nchunks; ++i) {
+    const int slot = (int)(i & 1);
+    const size_t bytes = nb(i);
+    const size_t ntr = bytes / tbytes;
+    if (i >= 2) HIP_TRY(hipEventSynchronize(pp.h2d[slot]));  // pinned input slot free
+    std::memcpy(pp.pin_in[slot], src + i * cbytes, bytes);
+    if (i >= 2) HIP_TRY(hipStreamWaitEvent(pp.s[0], pp.ex[slot], 0));  // device input slot free
+    HIP_TRY(hipMemcpyAsync(pp.d_in[slot], pp.pin_in[slot], bytes, hipMemcpyHostToDevice, pp.s[0]));
+    HIP_TRY(hipEventRecord(pp.h2d[slot], pp.s[0]));
+    HIP_TRY(hipStreamWaitEvent(pp.s[1], pp.h2d[slot], 0));
+    if (i >= 2) HIP_TRY(hipStreamWaitEvent(pp.s[1], pp.d2h[slot], 0));  // device output slot free
+    Plan* plan = nullptr;
+    if (get_plan(rank, dims, ntr, 0u, PREC_F32, false, &plan)) return -1;
+    if (run_plan(*plan, pp.d_in[slot], pp.d_out[slot], inverse, pp.s[1])) return -1;
+    HIP_TRY(hipEventRecord(pp.ex[slot], pp.s[1]));
+    if (i >= 2 && drain(i - 2)) return -1;  // pinned output slot free
+    HIP_TRY(hipStreamWaitEvent(pp.s[2], pp.ex[slot], 0));
+    HIP_TRY(hipMemcpyAsync(pp.pin_out[slot], pp.d_out[slot], bytes, hipMemcpyDeviceToHost, pp.s[2]));
+    HIP_TRY(hipEventRecord(pp.d2h[slot], pp.s[2]));
+  }
+  for (size_t i = nchunks >= 2 ? nchunks - 2 : 0; i < nchunks; ++i)
+    if (drain(i)) return -1;
   auto t1 = clk::now();
-  if (rc) return rc;
   if (t) {
     *t = Timing();
     t->exec_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

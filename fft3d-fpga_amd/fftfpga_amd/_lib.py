@@ -8,6 +8,7 @@ or CPU fallback for any transform.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
 
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -70,6 +71,25 @@ SIGNATURES = {
 _lib = None
 
 
+def _share_torch_hip_runtime() -> None:
+    """Map torch's bundled HIP runtime before libfftfpga.so.
+
+    libfftfpga.so needs libamdhip64.so.7.  If PyTorch (which bundles its own
+    copy, also with soname libamdhip64.so.7) is installed, loading that copy
+    first makes both resolve to one HIP runtime; two runtimes in one process
+    cannot both own the GPU.  Set FFTAMD_SYSTEM_HIP=1 to keep the system
+    runtime (C/Python processes that never use torch).
+    """
+    if os.environ.get("FFTAMD_SYSTEM_HIP") == "1":
+        return
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    hip = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(hip):
+        ctypes.CDLL(hip, mode=ctypes.RTLD_GLOBAL)
+
+
 def load(path: str | None = None) -> ctypes.CDLL:
     """Load the native library once and attach the declared signatures."""
     global _lib
@@ -80,6 +100,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         raise RuntimeError(
             f"native library {p} is missing; build it with `make -C fft3d-fpga_amd` "
             "(there is no fallback implementation)")
+    _share_torch_hip_runtime()
     lib = ctypes.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

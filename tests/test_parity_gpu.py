@@ -91,8 +91,10 @@ def test_3d_256_round_trip_and_parseval(gpu_lib):
     X, _ = _3d(gpu_lib, N, x, False)
     y, _ = _3d(gpu_lib, N, X, True)
     err = np.linalg.norm(y.astype(np.complex128) / N ** 3 - x) / np.linalg.norm(x)
-    e_in = np.vdot(x, x).real
-    e_out = np.vdot(X.astype(np.complex128), X.astype(np.complex128)).real
+    x64 = x.astype(np.complex128)
+    X64 = X.astype(np.complex128)
+    e_in = np.vdot(x64, x64).real
+    e_out = np.vdot(X64, X64).real
     print(f"256^3 round trip rel={err:.3e} parseval={(e_out / (N ** 3 * e_in) - 1):.2e}")
     assert err <= TOL
     assert abs(e_out / (N ** 3 * e_in) - 1) < 1e-5
@@ -186,13 +188,16 @@ def test_1d_double_vs_oracle(gpu_lib, orc, N, batch):
 
 # ---- batch, svm and bram entry points ---------------------------------------------------------
 
-def test_3d_ddr_batch(gpu_lib, orc):
-    N, B = 32, 3
-    x = orc.uniform(B * N ** 3, 9)
+@pytest.mark.parametrize("N,B,inv", [(32, 3, False), (64, 40, True), (256, 3, False)])
+def test_3d_ddr_batch(gpu_lib, orc, N, B, inv):
+    """Pipelined batch: 64 MiB chunks, so (64, 40) ends on a partial chunk and
+    (256, 3) runs three single-transform chunks through both pipeline slots."""
+    x = orc.uniform(B * N ** 3, 9 + N)
     out = np.zeros_like(x)
-    t = gpu_lib.fftfpgaf_c2c_3d_ddr_batch(N, x, out, False, False, B)
+    t = gpu_lib.fftfpgaf_c2c_3d_ddr_batch(N, x, out, inv, False, B)
     assert t.valid and t.exec_t > 0
-    err = orc.rel_l2(out, orc.fftnd(x, [N] * 3, batch=B))
+    err = orc.rel_l2(out, orc.fftnd(x, [N] * 3, batch=B, inverse=inv))
+    print(f"3d batch N={N} B={B} rel_l2={err:.3e} wall_ms={t.exec_t:.3f}")
     assert err <= TOL
 
 
