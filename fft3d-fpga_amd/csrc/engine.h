@@ -42,6 +42,12 @@ int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* h
 int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
             unsigned flags, int prec, void* stream);
 
+// FP32 device transform with real weights (one per point of a transform,
+// broadcast over the batch) fused into the first pass's load (w_load) and the
+// last pass's store (w_store); either may be null.
+int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din, void* dout,
+                     bool inverse, const float* w_load, const float* w_store, void* stream);
+
 // Host-only plan description (see engine.hip); returns the pass count or -1.
 int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
                   long long* out, int cap);

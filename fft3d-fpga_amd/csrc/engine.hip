@@ -68,6 +68,7 @@ struct Plan {
   int prec = PREC_F32;
   std::vector<PassDesc> passes;
   size_t total = 0;  // points including batch
+  size_t per = 0;    // points of one transform
   bool final_copy = false;
   bool needs_tmp = false;
   void* tmp = nullptr;
@@ -259,6 +260,7 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   size_t tot = 1;
   for (int d = 0; d < rank; ++d) tot *= dims[d];
   plan.total = tot * batch;
+  plan.per = tot;
   const bool bitrev = (flags & FLAG_BITREV_1D) && rank == 1;
   std::vector<PassDesc> ps;
 
@@ -441,14 +443,23 @@ static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, 
   return 0;
 }
 
-static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipStream_t stream) {
+static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipStream_t stream,
+                    const float* w_load = nullptr, const float* w_store = nullptr) {
+  if ((w_load || w_store) && plan.passes.empty()) {
+    set_error("weighted transform of a single point is not supported");
+    return -1;
+  }
   const size_t esz = plan.prec == PREC_F32 ? 8 : 16;
   auto ptr = [&](int b) -> void* {
     return b == B_IN ? const_cast<void*>(din) : (b == B_OUT ? dout : plan.tmp);
   };
   int cur = B_IN;
-  for (auto& p : plan.passes) {
+  for (size_t pi = 0; pi < plan.passes.size(); ++pi) {
+    const PassDesc& p = plan.passes[pi];
     PassArgs a{};
+    a.w_load = pi == 0 ? w_load : nullptr;
+    a.w_store = pi + 1 == plan.passes.size() ? w_store : nullptr;
+    a.w_mask = (unsigned long long)plan.per - 1ULL;
     a.in = ptr(p.src);
     a.out = ptr(p.dst);
     a.in_p = p.in_p;
@@ -885,6 +896,22 @@ int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* h
     t->exec_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   }
   return 0;
+}
+
+int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din, void* dout,
+                     bool inverse, const float* w_load, const float* w_store, void* stream) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (runtime_ensure_device()) return -1;
+  if (!check_dims(rank, dims, batch)) return -1;
+  if (!din || !dout) {
+    set_error("null device pointer");
+    return -1;
+  }
+  Plan* plan = nullptr;
+  if (get_plan(rank, dims, batch, 0u, PREC_F32, din == dout, &plan)) return -1;
+  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : st.stream;
+  return run_plan(*plan, din, dout, inverse, s, w_load, w_store);
 }
 
 int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,

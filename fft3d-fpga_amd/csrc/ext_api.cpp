@@ -38,6 +38,14 @@ int fftamd_z2z_dev(int rank, const unsigned* dims, unsigned batch, const double2
   return fftamd::dev_c2c(rank, d, batch, d_in, d_out, inverse != 0, flags, 1, stream);
 }
 
+int fftamd_c2c_weighted_dev(int rank, const unsigned* dims, unsigned batch, const float2* d_in,
+                            float2* d_out, int inverse, const float* w_in, const float* w_out,
+                            void* stream) {
+  size_t d[3];
+  if (!dims_ok(rank, dims, batch, d)) return -1;
+  return fftamd::dev_c2c_weighted(rank, d, batch, d_in, d_out, inverse != 0, w_in, w_out, stream);
+}
+
 int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2* d_in, float2* d_out, bool inv,
                             unsigned how_many, void* stream) {
   if (d_in == nullptr || d_out == nullptr || N == 0 || (N & (N - 1)) != 0 || how_many == 0) {

@@ -24,6 +24,12 @@ struct PassArgs {
   unsigned ptw_mmask;
   unsigned ptw_cshift;
   int inverse;
+  // optional real weights, one per point of a transform (broadcast over the
+  // batch): the input is multiplied by w_load[e & w_mask] when read, the
+  // output by w_store[e & w_mask] when written (e = element offset)
+  const float* w_load;
+  const float* w_store;
+  unsigned long long w_mask;
 };
 
 // Launches one pass; returns 0, -1 for an unsupported (prec, lg, mode, epi), -2 on a launch error.

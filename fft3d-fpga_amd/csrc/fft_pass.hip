@@ -133,12 +133,10 @@ __global__ void __launch_bounds__(geom_threads(L, MODE))
   const bool active = col < a.ncol;
   const unsigned ko = a.kbits ? brev_bits(kk, a.kbits) : kk;
 
-  const cpx<R>* __restrict__ in =
-      reinterpret_cast<const cpx<R>*>(a.in) +
-      ((long long)o * a.in_o + (long long)kk * a.in_k + (long long)col * a.in_c);
-  cpx<R>* __restrict__ out = reinterpret_cast<cpx<R>*>(a.out) +
-                             ((long long)o * a.out_o + (long long)ko * a.out_k +
-                              (long long)col * a.out_c);
+  const long long in_off = (long long)o * a.in_o + (long long)kk * a.in_k + (long long)col * a.in_c;
+  const long long out_off = (long long)o * a.out_o + (long long)ko * a.out_k + (long long)col * a.out_c;
+  const cpx<R>* __restrict__ in = reinterpret_cast<const cpx<R>*>(a.in) + in_off;
+  cpx<R>* __restrict__ out = reinterpret_cast<cpx<R>*>(a.out) + out_off;
   const cpx<R>* __restrict__ tw = reinterpret_cast<const cpx<R>*>(a.tw);
 
   cpx<R> v[PPT];
@@ -153,7 +151,10 @@ __global__ void __launch_bounds__(geom_threads(L, MODE))
       for (int r = 0; r < RAD; ++r) {
         const long long p = j + r * (L / RAD);
         cpx<R> x = {R(0), R(0)};
-        if (active) x = in[p * a.in_p];
+        if (active) {
+          x = in[p * a.in_p];
+          if (a.w_load) x = cscale(x, (R)a.w_load[(unsigned long long)(in_off + p * a.in_p) & a.w_mask]);
+        }
         v[b * RAD + r] = a.inverse ? cswap(x) : x;
       }
     }
@@ -180,7 +181,9 @@ __global__ void __launch_bounds__(geom_threads(L, MODE))
     for (int m = 0; m < PPT; ++m) {
       const int q = tau + m * TPL;
       cpx<R> x = lds[G::idx(c, (int)brev_bits((unsigned)q, LG))];
-      if (active) out[(long long)q * a.out_p] = a.inverse ? cswap(x) : x;
+      if (a.inverse) x = cswap(x);
+      if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + (long long)q * a.out_p) & a.w_mask]);
+      if (active) out[(long long)q * a.out_p] = x;
     }
   } else {
 #pragma unroll
@@ -197,7 +200,11 @@ __global__ void __launch_bounds__(geom_threads(L, MODE))
           const cpx<R> wh = reinterpret_cast<const cpx<R>*>(a.ptw_hi)[t >> a.ptw_hb];
           x = cmul(x, cmul(wh, wl));
         }
-        if (active) out[(long long)k * a.out_p] = a.inverse ? cswap(x) : x;
+        if (a.inverse) x = cswap(x);
+        if (active) {
+          if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + (long long)k * a.out_p) & a.w_mask]);
+          out[(long long)k * a.out_p] = x;
+        }
       }
     }
   }

@@ -34,6 +34,18 @@ int fftamd_c2c_dev(int rank, const unsigned *dims, unsigned batch, const float2 
 int fftamd_z2z_dev(int rank, const unsigned *dims, unsigned batch, const double2 *d_in,
                    double2 *d_out, int inverse, unsigned flags, void *stream);
 
+/* Weighted single-precision transform: out = w_out .* F(w_in .* in), where F is
+ * the forward (inverse == 0) or unnormalised backward transform and w_in,
+ * w_out are real FP32 device arrays of one transform's shape (prod(dims)
+ * points, broadcast over the batch), either may be NULL.  The weights are
+ * applied inside the first pass's loads and the last pass's stores, so they
+ * cost no extra pass over memory.  With w_in = W and inverse = 1 this is the
+ * adjoint of out = W .* F(in), which the autograd binding uses as its
+ * backward. */
+int fftamd_c2c_weighted_dev(int rank, const unsigned *dims, unsigned batch, const float2 *d_in,
+                            float2 *d_out, int inverse, const float *w_in, const float *w_out,
+                            void *stream);
+
 /* Device-pointer twin of fftfpgaf_c2c_3d_ddr / _3d_ddr_batch: how_many
  * contiguous N^3 transforms (how_many >= 1). */
 int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2 *d_in, float2 *d_out, bool inv,
