@@ -38,7 +38,7 @@ int host_c2c_mapped(int rank, const size_t* dims, size_t batch, const void* hin,
 // Host pointers, `count` transforms pipelined in chunks across copy and compute streams.
 int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* hin, void* hout,
                        bool inverse, Timing* t);
-// Device pointers on the caller's stream (nullptr = the library stream); asynchronous.
+// Device pointers on the caller's stream (nullptr = the HIP null stream); asynchronous.
 int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
             unsigned flags, int prec, void* stream);
 

@@ -910,7 +910,7 @@ int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din
   }
   Plan* plan = nullptr;
   if (get_plan(rank, dims, batch, 0u, PREC_F32, din == dout, &plan)) return -1;
-  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : st.stream;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
   return run_plan(*plan, din, dout, inverse, s, w_load, w_store);
 }
 
@@ -927,7 +927,7 @@ int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* d
   Plan* plan = nullptr;
   const bool aliased = din == dout;
   if (get_plan(rank, dims, batch, flags, prec, aliased, &plan)) return -1;
-  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : st.stream;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
   return run_plan(*plan, din, dout, inverse, s);
 }
 

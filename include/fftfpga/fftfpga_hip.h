@@ -5,8 +5,8 @@
  * The reference has no counterpart for these entry points: it only accepts
  * host pointers and copies over PCIe on every call (api/src/fft3d.c:242-264).
  * They run the same gfx950 kernels as fftfpga.h, asynchronously on the
- * caller's HIP stream (a hipStream_t passed as void*; NULL selects the
- * library's own stream).  All return 0 on success and a negative code on
+ * caller's HIP stream (a hipStream_t passed as void*; NULL is the HIP null
+ * stream, as everywhere in HIP).  All return 0 on success and a negative code on
  * error, with the reason in fftamd_last_error().
  *
  * Layout conventions are those of fftfpga.h: interleaved complex, row-major,

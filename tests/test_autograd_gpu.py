@@ -49,15 +49,20 @@ def test_adjoint_vs_fp64_finite_differences(gpu_lib, B, N, ndir):
     loss.backward()
     g = xr.grad.to(torch.complex128)
     gen = torch.Generator(device="cuda").manual_seed(1234)
-    worst = 0.0
     h = 1.0  # L is quadratic: the central difference has no truncation error
+    fds, dds = [], []
     for _ in range(ndir):
         v = torch.randn(x.shape, dtype=torch.complex128, generator=gen, device="cuda")
-        fd = (_loss64(x + h * v, w) - _loss64(x - h * v, w)) / (2 * h)
-        dd = (torch.conj(g) * v).real.sum()
-        worst = max(worst, abs((dd - fd) / fd).item())
-    print(f"adjoint FD B={B} N={N} worst rel={worst:.3e}")
-    assert worst < 1e-5
+        fds.append(((_loss64(x + h * v, w) - _loss64(x - h * v, w)) / (2 * h)).item())
+        dds.append((torch.conj(g) * v).real.sum().item())
+    fds, dds = np.array(fds), np.array(dds)
+    # errors are measured against the RMS directional derivative: single random
+    # directions can be nearly orthogonal to the gradient (derivative ~ 0)
+    scale = np.sqrt(np.mean(fds ** 2))
+    worst = np.max(np.abs(dds - fds)) / scale
+    med = np.median(np.abs(dds - fds) / np.abs(fds))
+    print(f"adjoint FD B={B} N={N} worst/rms={worst:.3e} median rel={med:.3e}")
+    assert worst < 2e-6 and med < 2e-6
 
 
 def test_gradient_matches_torch_fft_autograd(gpu_lib):
