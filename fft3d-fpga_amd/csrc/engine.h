@@ -48,6 +48,21 @@ int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* d
 int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din, void* dout,
                      bool inverse, const float* w_load, const float* w_store, void* stream);
 
+// ---- multi-transform plans (uneven sizes, one persistent launch) ---------------
+int multi_create(void** handle, unsigned count, const unsigned* dims3, const unsigned long long* in_off,
+                 const unsigned long long* out_off, const void* din, void* dout, bool inverse);
+int multi_exec(void* handle, void* stream);
+int multi_check(void* handle, void* stream);
+void multi_destroy(void* handle);
+int multi_tune(void* handle, void* stream, const char* cache_path, int reps, double* best_us);
+int multi_get_schedule(void* handle, int* mode, unsigned* grid, unsigned* chunk, unsigned* items,
+                       unsigned* nphase);
+int multi_set_schedule(void* handle, int mode, unsigned grid, unsigned chunk);
+void multi_set_spin_limit(unsigned polls);
+int graph_capture_multi(void** out, void** handles, int n, void* stream);
+int graph_launch(void* graph, void* stream);
+void graph_destroy(void* graph);
+
 // Host-only plan description (see engine.hip); returns the pass count or -1.
 int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
                   long long* out, int cap);

@@ -8,18 +8,24 @@
 // once in FP64 and stored in the transform precision.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <fstream>
+#include <numeric>
+#include <sstream>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
 
 #include "engine.h"
+#include "fft_multi.h"
 #include "fft_pass.h"
 #include "fft_schedule.h"
 
@@ -519,6 +525,356 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
   }
   return 0;
 }
+
+
+static bool check_dims(int rank, const size_t* dims, size_t batch);
+static double ms_between(hipEvent_t a, hipEvent_t b);
+
+// ---------------------------------------------------------------------------
+// multi-transform plans: transforms of uneven sizes in one persistent launch
+// ---------------------------------------------------------------------------
+struct MultiPlan {
+  std::vector<MultiPass> passes;
+  std::vector<MultiItem> items;
+  std::vector<unsigned> phase_start;
+  unsigned nphase = 0;
+  MultiPass* d_passes = nullptr;
+  MultiItem* d_items = nullptr;
+  unsigned* d_phase = nullptr;
+  MultiCtl* d_ctl = nullptr;
+  unsigned grid = 0, chunk = 4;
+  int mode = 0;  // 0: one launch, grid barriers between phases; 1: one launch per phase
+  std::string sig;
+  size_t points = 0;
+  ~MultiPlan() {
+    if (d_passes) (void)hipFree(d_passes);
+    if (d_items) (void)hipFree(d_items);
+    if (d_phase) (void)hipFree(d_phase);
+    if (d_ctl) (void)hipFree(d_ctl);
+  }
+};
+
+static unsigned g_multi_spin_limit = 1u << 22;
+
+static int multi_default_grid(unsigned* grid, unsigned* per_cu_max) {
+  int ncu = 0, per = 0;
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, S().dev < 0 ? 0 : S().dev));
+  if (multi_max_blocks_per_cu(&per) || per <= 0) {
+    set_error("occupancy query for the multi-transform kernel failed");
+    return -1;
+  }
+  *per_cu_max = (unsigned)per;
+  *grid = (unsigned)(ncu * per);
+  return 0;
+}
+
+int multi_create(void** handle, unsigned count, const unsigned* dims3, const unsigned long long* in_off,
+                 const unsigned long long* out_off, const void* din, void* dout, bool inverse) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (runtime_ensure_device()) return -1;
+  if (!handle || count == 0 || !dims3 || !in_off || !out_off || !din || !dout) {
+    set_error("bad multi-transform arguments");
+    return -1;
+  }
+  auto mp = std::make_unique<MultiPlan>();
+  struct Tr {
+    unsigned idx;
+    size_t pts;
+    std::vector<PassDesc> ps;
+  };
+  std::vector<Tr> trs(count);
+  std::map<std::string, unsigned> sizes;
+  for (unsigned i = 0; i < count; ++i) {
+    size_t d[3] = {dims3[3 * i], dims3[3 * i + 1], dims3[3 * i + 2]};
+    if (!check_dims(3, d, 1)) return -1;
+    for (int a = 0; a < 3; ++a)
+      if (d[a] > (1u << MULTI_MAX_LG)) {
+        set_error("multi-transform sizes are limited to 256 points per axis");
+        return -1;
+      }
+    Plan p;
+    const bool aliased = (const char*)din + 8 * in_off[i] == (const char*)dout + 8 * out_off[i];
+    if (build_plan(3, d, 1, 0u, PREC_F32, aliased, p)) return -1;
+    if (p.needs_tmp || p.final_copy) {
+      set_error("multi-transform plan needs a temporary buffer");
+      return -1;
+    }
+    trs[i] = {i, p.total, p.passes};
+    std::ostringstream k;
+    k << d[0] << "x" << d[1] << "x" << d[2];
+    sizes[k.str()] += 1;
+    mp->points += p.total;
+  }
+  {
+    std::ostringstream sg;
+    sg << "multi3d";
+    for (auto& kv : sizes) sg << ":" << kv.first << "*" << kv.second;
+    mp->sig = sg.str();
+  }
+  // largest transforms first within every phase
+  std::vector<unsigned> order(count);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](unsigned a, unsigned b) { return trs[a].pts > trs[b].pts; });
+  unsigned nphase = 0;
+  for (auto& t : trs) nphase = std::max<unsigned>(nphase, (unsigned)t.ps.size());
+  if (nphase > (unsigned)MULTI_MAX_PHASES) {
+    set_error("too many phases");
+    return -1;
+  }
+  mp->nphase = nphase;
+  // passes, indexed [phase][transform in order]
+  std::vector<std::vector<unsigned>> pass_of(nphase, std::vector<unsigned>(count, ~0u));
+  for (unsigned ph = 0; ph < nphase; ++ph) {
+    for (unsigned oi = 0; oi < count; ++oi) {
+      const unsigned i = order[oi];
+      if (ph >= trs[i].ps.size()) continue;
+      const PassDesc& p = trs[i].ps[ph];
+      const int L = 1 << p.lg;
+      PassArgs a{};
+      const char* ib = (const char*)din + 8 * in_off[i];
+      char* ob = (char*)dout + 8 * out_off[i];
+      a.in = p.src == B_IN ? (const void*)ib : (const void*)ob;
+      a.out = p.dst == B_IN ? (void*)ib : (void*)ob;
+      a.in_p = p.in_p;
+      a.out_p = p.out_p;
+      a.in_c = p.in_c;
+      a.out_c = p.out_c;
+      a.in_k = p.in_k;
+      a.out_k = p.out_k;
+      a.in_o = p.in_o;
+      a.out_o = p.out_o;
+      a.ncol = p.ncol;
+      a.ncb = (p.ncol + geom_cols(L, p.mode) - 1) / geom_cols(L, p.mode);
+      a.K = p.K;
+      a.kbits = p.kbits;
+      a.inverse = inverse ? 1 : 0;
+      if (sched_np(L) > 1 && get_stockham(PREC_F32, L, &a.tw)) return -1;
+      MultiPass m{};
+      m.a = a;
+      m.path = p.mode * MULTI_MAX_LG + (p.lg - 1);
+      pass_of[ph][oi] = (unsigned)mp->passes.size();
+      mp->passes.push_back(m);
+    }
+  }
+  mp->phase_start.push_back(0);
+  for (unsigned ph = 0; ph < nphase; ++ph) {
+    for (unsigned oi = 0; oi < count; ++oi) {
+      const unsigned pid = pass_of[ph][oi];
+      if (pid == ~0u) continue;
+      const PassArgs& a = mp->passes[pid].a;
+      const unsigned long long nb = (unsigned long long)a.ncb * a.K * trs[order[oi]].ps[ph].nouter;
+      for (unsigned long long b = 0; b < nb; ++b) mp->items.push_back({pid, (unsigned)b});
+    }
+    mp->phase_start.push_back((unsigned)mp->items.size());
+  }
+  HIP_TRY(hipMalloc(&mp->d_passes, sizeof(MultiPass) * mp->passes.size()));
+  HIP_TRY(hipMalloc(&mp->d_items, sizeof(MultiItem) * std::max<size_t>(1, mp->items.size())));
+  HIP_TRY(hipMalloc(&mp->d_phase, sizeof(unsigned) * mp->phase_start.size()));
+  HIP_TRY(hipMalloc(&mp->d_ctl, sizeof(MultiCtl)));
+  HIP_TRY(hipMemcpy(mp->d_passes, mp->passes.data(), sizeof(MultiPass) * mp->passes.size(), hipMemcpyHostToDevice));
+  if (!mp->items.empty())
+    HIP_TRY(hipMemcpy(mp->d_items, mp->items.data(), sizeof(MultiItem) * mp->items.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(mp->d_phase, mp->phase_start.data(), sizeof(unsigned) * mp->phase_start.size(),
+                    hipMemcpyHostToDevice));
+  unsigned per = 0;
+  if (multi_default_grid(&mp->grid, &per)) return -1;
+  *handle = mp.release();
+  return 0;
+}
+
+int multi_exec(void* handle, void* stream) {
+  MultiPlan* mp = static_cast<MultiPlan*>(handle);
+  if (!mp) {
+    set_error("null multi-transform handle");
+    return -1;
+  }
+  if (mp->nphase == 0) return 0;
+  if (mp->mode == 0) {
+    if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, 0, mp->nphase, mp->d_ctl, mp->grid, mp->chunk,
+                     g_multi_spin_limit, stream)) {
+      set_error("multi-transform launch failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+  } else {
+    for (unsigned ph = 0; ph < mp->nphase; ++ph)
+      if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, ph, 1, mp->d_ctl, mp->grid, mp->chunk,
+                       g_multi_spin_limit, stream)) {
+        set_error("multi-transform launch failed: %s", hipGetErrorString(hipGetLastError()));
+        return -1;
+      }
+  }
+  return 0;
+}
+
+int multi_check(void* handle, void* stream) {
+  MultiPlan* mp = static_cast<MultiPlan*>(handle);
+  MultiCtl c{};
+  HIP_TRY(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+  HIP_TRY(hipMemcpy(&c, mp->d_ctl, sizeof(c), hipMemcpyDeviceToHost));
+  if (c.err) {
+    set_error("multi-transform grid barrier wait exceeded its bound");
+    return -1;
+  }
+  return 0;
+}
+
+void multi_destroy(void* handle) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  delete static_cast<MultiPlan*>(handle);
+}
+
+int multi_get_schedule(void* handle, int* mode, unsigned* grid, unsigned* chunk, unsigned* items,
+                       unsigned* nphase) {
+  MultiPlan* mp = static_cast<MultiPlan*>(handle);
+  if (!mp) return -1;
+  if (mode) *mode = mp->mode;
+  if (grid) *grid = mp->grid;
+  if (chunk) *chunk = mp->chunk;
+  if (items) *items = (unsigned)mp->items.size();
+  if (nphase) *nphase = mp->nphase;
+  return 0;
+}
+
+int multi_set_schedule(void* handle, int mode, unsigned grid, unsigned chunk) {
+  MultiPlan* mp = static_cast<MultiPlan*>(handle);
+  unsigned g = 0, per = 0;
+  if (!mp || multi_default_grid(&g, &per)) return -1;
+  if ((mode != 0 && mode != 1) || chunk == 0 || grid == 0 || grid > g) {
+    set_error("schedule out of range (grid must be 1..%u resident workgroups)", g);
+    return -1;
+  }
+  mp->mode = mode;
+  mp->grid = grid;
+  mp->chunk = chunk;
+  return 0;
+}
+
+void multi_set_spin_limit(unsigned polls) { g_multi_spin_limit = polls ? polls : 1u; }
+
+// Autotuner: time every (mode, workgroups per CU, chunk) candidate on the
+// plan's own buffers, keep the fastest, and record it under the plan's shape
+// signature in a text cache (one line: signature mode per_cu chunk microseconds).
+int multi_tune(void* handle, void* stream, const char* cache_path, int reps, double* best_us) {
+  MultiPlan* mp = static_cast<MultiPlan*>(handle);
+  if (!mp) return -1;
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  unsigned gmax = 0, per = 0;
+  if (multi_default_grid(&gmax, &per)) return -1;
+  const unsigned ncu = gmax / per;
+  // cached?
+  if (cache_path && *cache_path) {
+    std::ifstream f(cache_path);
+    std::string line;
+    while (std::getline(f, line)) {
+      std::istringstream ls(line);
+      std::string sg;
+      int mode;
+      unsigned pc, ch;
+      double us;
+      if ((ls >> sg >> mode >> pc >> ch >> us) && sg == mp->sig && pc >= 1 && pc <= per) {
+        mp->mode = mode;
+        mp->grid = pc * ncu;
+        mp->chunk = ch;
+        if (best_us) *best_us = us;
+        return 1;  // from cache
+      }
+    }
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double best = 1e30;
+  int bmode = 0;
+  unsigned bpc = per, bch = 4;
+  const unsigned chunks[] = {1, 2, 4, 8, 16};
+  for (int mode = 0; mode <= 1; ++mode)
+    for (unsigned pc = 1; pc <= per; ++pc)
+      for (unsigned ch : chunks) {
+        mp->mode = mode;
+        mp->grid = pc * ncu;
+        mp->chunk = ch;
+        if (multi_exec(mp, s) || multi_check(mp, s)) {  // warm-up and error check
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          return -1;
+        }
+        HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < std::max(1, reps); ++r)
+          if (multi_exec(mp, s)) return -1;
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        const double us = 1e3 * ms_between(e0, e1) / std::max(1, reps);
+        if (us < best) {
+          best = us;
+          bmode = mode;
+          bpc = pc;
+          bch = ch;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (multi_check(mp, s)) return -1;
+  mp->mode = bmode;
+  mp->grid = bpc * ncu;
+  mp->chunk = bch;
+  if (best_us) *best_us = best;
+  if (cache_path && *cache_path) {
+    std::ofstream f(cache_path, std::ios::app);
+    if (f) f << mp->sig << " " << bmode << " " << bpc << " " << bch << " " << best << "\n";
+  }
+  return 0;
+}
+
+// ---- graphs: a sequence of multi-transform executions captured once ------------
+struct Graph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  ~Graph() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+int graph_capture_multi(void** out, void** handles, int n, void* stream) {
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!out || !handles || n <= 0 || s == nullptr) {
+    set_error("graph capture needs handles and a non-null stream");
+    return -1;
+  }
+  auto gr = std::make_unique<Graph>();
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int i = 0; i < n && !rc; ++i) rc = multi_exec(handles[i], s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return -1;
+  }
+  if (e != hipSuccess) {
+    set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    return -1;
+  }
+  gr->g = g;
+  HIP_TRY(hipGraphInstantiate(&gr->x, g, nullptr, nullptr, 0));
+  *out = gr.release();
+  return 0;
+}
+
+int graph_launch(void* graph, void* stream) {
+  Graph* g = static_cast<Graph*>(graph);
+  if (!g) return -1;
+  HIP_TRY(hipGraphLaunch(g->x, reinterpret_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+void graph_destroy(void* graph) { delete static_cast<Graph*>(graph); }
 
 // ---------------------------------------------------------------------------
 // runtime

@@ -1,6 +1,8 @@
 // ext_api.cpp -- device-pointer extension entry points (fftfpga_hip.h).
 #include "fftfpga/fftfpga_hip.h"
 
+#include <vector>
+
 #include "engine.h"
 
 namespace {
@@ -55,6 +57,68 @@ int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2* d_in, float2* d_out, bool 
   size_t d[3] = {N, N, N};
   return fftamd::dev_c2c(3, d, how_many, d_in, d_out, inv, 0u, 0, stream);
 }
+
+int fftamd_multi_create(fftamd_multi_t* plan, unsigned count, const fftamd_xform3d* xf,
+                        const float2* d_in, float2* d_out, int inverse) {
+  if (!plan || !xf || count == 0) {
+    fftamd::set_error("bad multi-transform arguments");
+    return -1;
+  }
+  std::vector<unsigned> dims(3 * (size_t)count);
+  std::vector<unsigned long long> io(count), oo(count);
+  for (unsigned i = 0; i < count; ++i) {
+    if (xf[i].reserved != 0) {
+      fftamd::set_error("fftamd_xform3d.reserved must be 0");
+      return -1;
+    }
+    for (int a = 0; a < 3; ++a) dims[3 * i + a] = xf[i].n[a];
+    io[i] = xf[i].in_offset;
+    oo[i] = xf[i].out_offset;
+  }
+  void* h = nullptr;
+  if (fftamd::multi_create(&h, count, dims.data(), io.data(), oo.data(), d_in, d_out, inverse != 0))
+    return -1;
+  *plan = reinterpret_cast<fftamd_multi_t>(h);
+  return 0;
+}
+
+int fftamd_multi_exec(fftamd_multi_t plan, void* stream) { return fftamd::multi_exec(plan, stream); }
+
+int fftamd_multi_check(fftamd_multi_t plan, void* stream) { return fftamd::multi_check(plan, stream); }
+
+int fftamd_multi_tune(fftamd_multi_t plan, void* stream, const char* cache_path, int reps,
+                      double* best_us) {
+  return fftamd::multi_tune(plan, stream, cache_path, reps, best_us);
+}
+
+int fftamd_multi_schedule(fftamd_multi_t plan, int* mode, unsigned* grid, unsigned* chunk,
+                          unsigned* items, unsigned* phases) {
+  return fftamd::multi_get_schedule(plan, mode, grid, chunk, items, phases);
+}
+
+int fftamd_multi_set_schedule(fftamd_multi_t plan, int mode, unsigned grid, unsigned chunk) {
+  return fftamd::multi_set_schedule(plan, mode, grid, chunk);
+}
+
+void fftamd_multi_destroy(fftamd_multi_t plan) { fftamd::multi_destroy(plan); }
+
+void fftamd_multi_set_spin_limit(unsigned polls) { fftamd::multi_set_spin_limit(polls); }
+
+int fftamd_graph_capture(fftamd_graph_t* graph, fftamd_multi_t* plans, int n, void* stream) {
+  if (!graph || !plans) {
+    fftamd::set_error("bad graph arguments");
+    return -1;
+  }
+  std::vector<void*> hs(plans, plans + (n > 0 ? n : 0));
+  void* g = nullptr;
+  if (fftamd::graph_capture_multi(&g, hs.data(), n, stream)) return -1;
+  *graph = reinterpret_cast<fftamd_graph_t>(g);
+  return 0;
+}
+
+int fftamd_graph_launch(fftamd_graph_t graph, void* stream) { return fftamd::graph_launch(graph, stream); }
+
+void fftamd_graph_destroy(fftamd_graph_t graph) { fftamd::graph_destroy(graph); }
 
 const char* fftamd_last_error(void) { return fftamd::last_error(); }
 

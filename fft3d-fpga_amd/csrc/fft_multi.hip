@@ -1,0 +1,141 @@
+// fft_multi.hip -- persistent kernel for a batch of transforms of uneven
+// power-of-two sizes (the GPU counterpart of the reference's batched 3D path,
+// api/src/fft3d.c:396-913, which handles one size per call).
+//
+// One launch runs every pass of every transform.  Pass p of all transforms
+// forms phase p.  Within a phase, 256-thread workgroups take chunks of tiles
+// from a device-side work queue (one atomic ticket per chunk; tiles are
+// ordered largest transform first), run them with the same pass_block() as
+// the one-pass kernels, and meet at a grid barrier before the next phase:
+// every wave publishes its stores (agent-scope release), one thread per
+// workgroup arrives on a counter and waits for all workgroups (agent-scope
+// acquire).  The wait is bounded: past spin_limit polls the workgroup sets
+// ctl->err and exits, so a launch that cannot make progress ends with an
+// error instead of hanging.  The host sizes the grid to what is resident.
+#include <hip/hip_runtime.h>
+
+#include "fft_multi.h"
+#include "fft_pass_dev.h"
+
+namespace fftamd {
+
+template <int MODE, int LG>
+__device__ __forceinline__ void run_path(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
+  static_assert(geom_threads(1 << LG, MODE) == MULTI_THREADS, "multi paths use 256-thread tiles");
+  pass_block<float, (1 << LG), MODE, EPI_NONE>(a, blk, lds);
+}
+
+__device__ __forceinline__ void dispatch_path(int path, const PassArgs& a, unsigned long long blk,
+                                              cpx<float>* lds) {
+  switch (path) {
+    case 0: run_path<MODE_CONTIG, 1>(a, blk, lds); break;
+    case 1: run_path<MODE_CONTIG, 2>(a, blk, lds); break;
+    case 2: run_path<MODE_CONTIG, 3>(a, blk, lds); break;
+    case 3: run_path<MODE_CONTIG, 4>(a, blk, lds); break;
+    case 4: run_path<MODE_CONTIG, 5>(a, blk, lds); break;
+    case 5: run_path<MODE_CONTIG, 6>(a, blk, lds); break;
+    case 6: run_path<MODE_CONTIG, 7>(a, blk, lds); break;
+    case 7: run_path<MODE_CONTIG, 8>(a, blk, lds); break;
+    case 8: run_path<MODE_STRIDED, 1>(a, blk, lds); break;
+    case 9: run_path<MODE_STRIDED, 2>(a, blk, lds); break;
+    case 10: run_path<MODE_STRIDED, 3>(a, blk, lds); break;
+    case 11: run_path<MODE_STRIDED, 4>(a, blk, lds); break;
+    case 12: run_path<MODE_STRIDED, 5>(a, blk, lds); break;
+    case 13: run_path<MODE_STRIDED, 6>(a, blk, lds); break;
+    case 14: run_path<MODE_STRIDED, 7>(a, blk, lds); break;
+    case 15: run_path<MODE_STRIDED, 8>(a, blk, lds); break;
+    default: break;
+  }
+}
+
+// largest LDS footprint over the 16 paths (L <= 256, 256 threads)
+constexpr int multi_lds_elems() {
+  int m = 1;
+  for (int lg = 1; lg <= MULTI_MAX_LG; ++lg) {
+    const int L = 1 << lg;
+    const int c0 = geom_cols(L, MODE_CONTIG) * (L + L / 16);
+    const int c1 = L * (geom_cols(L, MODE_STRIDED) + 1);
+    m = c0 > m ? c0 : m;
+    m = c1 > m ? c1 : m;
+  }
+  return m;
+}
+
+__global__ void k_multi_reset(MultiCtl* ctl) {
+  const unsigned i = threadIdx.x;
+  unsigned* w = reinterpret_cast<unsigned*>(ctl);
+  if (i < sizeof(MultiCtl) / sizeof(unsigned)) w[i] = 0u;
+}
+
+// 4 waves per SIMD: the 34.8 KB LDS footprint allows 4 workgroups per CU
+__global__ void __launch_bounds__(MULTI_THREADS, 4)
+    k_fft_multi(const MultiPass* __restrict__ passes, const MultiItem* __restrict__ items,
+                const unsigned* __restrict__ phase_start, unsigned phase0, unsigned nphase,
+                MultiCtl* ctl, unsigned chunk, unsigned spin_limit) {
+  __shared__ cpx<float> lds[multi_lds_elems()];
+  __shared__ unsigned s_ticket;
+  __shared__ int s_abort;
+  const unsigned nwg = gridDim.x;
+  for (unsigned ph = 0; ph < nphase; ++ph) {
+    const unsigned gph = phase0 + ph;
+    const unsigned base = phase_start[gph];
+    const unsigned count = phase_start[gph + 1] - base;
+    for (;;) {
+      __syncthreads();  // the previous tile's LDS reads and s_ticket reads are done
+      if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl->head[gph], chunk);
+      __syncthreads();
+      const unsigned t0 = s_ticket;
+      if (t0 >= count) break;
+      const unsigned t1 = min(t0 + chunk, count);
+      for (unsigned t = t0; t < t1; ++t) {
+        if (t != t0) __syncthreads();
+        const MultiItem it = items[base + t];
+        const MultiPass* mp = passes + it.pass;
+        dispatch_path(mp->path, mp->a, it.block, lds);
+      }
+    }
+    if (ph + 1 < nphase) {
+      // grid barrier between phases
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(&ctl->barrier, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = (ph + 1) * nwg;
+        unsigned polls = 0;
+        int abort = 0;
+        while (__hip_atomic_load(&ctl->barrier, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          if (++polls > spin_limit) {
+            __hip_atomic_store(&ctl->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            abort = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+        s_abort = abort;
+      }
+      __syncthreads();
+      if (s_abort) return;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+  }
+}
+
+int launch_multi(const MultiPass* passes, const MultiItem* items, const unsigned* phase_start,
+                 unsigned phase0, unsigned nphase, MultiCtl* ctl, unsigned grid, unsigned chunk,
+                 unsigned spin_limit, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_multi_reset, dim3(1), dim3(64), 0, s, ctl);
+  hipLaunchKernelGGL(k_fft_multi, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items, phase_start,
+                     phase0, nphase, ctl, chunk, spin_limit);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int multi_max_blocks_per_cu(int* out) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_multi, MULTI_THREADS, 0) != hipSuccess)
+    return -1;
+  *out = n;
+  return 0;
+}
+
+}  // namespace fftamd

@@ -8,206 +8,22 @@
 // strided axes are read directly in 128-byte row segments, so no separate
 // transpose buffer or bit-reversal kernel exists.
 //
-// Addressing of point p of line (o, k, col):
-//   in  + o*in_o  + k*in_k  + col*in_c  + p*in_p
-//   out + o*out_o + kk*out_k + col*out_c + p*out_p   (kk = bitrev(k) if kbits)
-// MODE_CONTIG: in_p == 1, threads of a line are adjacent (coalesced along p).
-// MODE_STRIDED: in_c == 1, threads of a row are adjacent (coalesced along col).
-//
-// Each workgroup loads its C lines straight from HBM into VGPRs (stage 0
-// reads the first stage's butterfly inputs directly), exchanges through LDS
-// between radix stages, and the last stage writes straight to HBM in natural
-// order.  LDS rows are padded so the strided exchanges avoid bank conflicts.
+// The tile work (addressing, radix stages, epilogues) is pass_block() in
+// fft_pass_dev.h; this file holds the one-tile-per-workgroup kernels and the
+// launch table over (precision, line length, mode, epilogue).
 #include <hip/hip_runtime.h>
 
-#include "fft_device.h"
-#include "fft_pass.h"
-#include "fft_schedule.h"
+#include "fft_pass_dev.h"
 
 namespace fftamd {
-
-template <int L, int MODE>
-struct Geom {
-  static constexpr int PPT = geom_ppt(L);
-  static constexpr int TPL = geom_tpl(L);
-  static constexpr int C = geom_cols(L, MODE);
-  static constexpr int NT = C * TPL;
-  static constexpr int NP = sched_np(L);
-  // contiguous: [line][p + p/16]; strided: [p][col] with one pad column
-  static constexpr int LP = L + L / 16;
-  static constexpr int LDS_ELEMS = MODE == MODE_CONTIG ? C * LP : L * (C + 1);
-  static __device__ __forceinline__ int idx(int c, int p) {
-    if constexpr (MODE == MODE_CONTIG) return c * LP + p + (p >> 4);
-    else return p * (C + 1) + c;
-  }
-};
-
-template <typename R, int L, int MODE, int S>
-__device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* __restrict__ tw) {
-  using G = Geom<L, MODE>;
-  constexpr int RAD = sched_radix(L, S);
-  constexpr int NS = sched_ns(L, S);
-  constexpr int NB = G::PPT / RAD;
-  if constexpr (S > 0) {
-    constexpr int OFF = sched_twoff(L, S);
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int j = tau + b * G::TPL;
-      const int k = j & (NS - 1);
-      const cpx<R>* t = tw + OFF + k * (RAD - 1);
-#pragma unroll
-      for (int r = 1; r < RAD; ++r) v[b * RAD + r] = cmul(v[b * RAD + r], t[r - 1]);
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < NB; ++b) dft<R, RAD>(v + b * RAD);
-}
-
-// write stage-S results to LDS at their Stockham destinations
-template <typename R, int L, int MODE, int S>
-__device__ __forceinline__ void stage_store_lds(const cpx<R>* v, cpx<R>* lds, int c, int tau) {
-  using G = Geom<L, MODE>;
-  constexpr int RAD = sched_radix(L, S);
-  constexpr int NS = sched_ns(L, S);
-  constexpr int NB = G::PPT / RAD;
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = tau + b * G::TPL;
-    const int base = (j / NS) * NS * RAD + (j & (NS - 1));
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) lds[G::idx(c, base + r * NS)] = v[b * RAD + r];
-  }
-}
-
-// read stage-S butterfly inputs from LDS
-template <typename R, int L, int MODE, int S>
-__device__ __forceinline__ void stage_load_lds(cpx<R>* v, const cpx<R>* lds, int c, int tau) {
-  using G = Geom<L, MODE>;
-  constexpr int RAD = sched_radix(L, S);
-  constexpr int NB = G::PPT / RAD;
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = tau + b * G::TPL;
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) v[b * RAD + r] = lds[G::idx(c, j + r * (L / RAD))];
-  }
-}
-
-template <typename R, int L, int MODE, int S>
-__device__ __forceinline__ void run_stages(cpx<R>* v, cpx<R>* lds, int c, int tau,
-                                           const cpx<R>* __restrict__ tw) {
-  using G = Geom<L, MODE>;
-  if constexpr (S < G::NP) {
-    if constexpr (S > 0) {
-      __syncthreads();  // previous stage's LDS writes are visible
-      stage_load_lds<R, L, MODE, S>(v, lds, c, tau);
-    }
-    stage_compute<R, L, MODE, S>(v, tau, tw);
-    if constexpr (S + 1 < G::NP) {
-      if constexpr (S > 0) __syncthreads();  // every thread finished reading this stage
-      stage_store_lds<R, L, MODE, S>(v, lds, c, tau);
-      run_stages<R, L, MODE, S + 1>(v, lds, c, tau, tw);
-    }
-  }
-}
 
 template <typename R, int L, int MODE, int EPI>
 __global__ void __launch_bounds__(geom_threads(L, MODE))
     k_fft_pass(const PassArgs a) {
   using G = Geom<L, MODE>;
-  constexpr int PPT = G::PPT, TPL = G::TPL, C = G::C;
-  constexpr int NP = G::NP;
-  constexpr bool NEED_LDS = NP > 1 || EPI == EPI_BITREV;
+  constexpr bool NEED_LDS = G::NP > 1 || EPI == EPI_BITREV;
   __shared__ cpx<R> lds[NEED_LDS ? G::LDS_ELEMS : 1];
-
-  const int tid = threadIdx.x;
-  const int c = MODE == MODE_CONTIG ? tid / TPL : tid % C;
-  const int tau = MODE == MODE_CONTIG ? tid % TPL : tid / C;
-
-  unsigned long long g = blockIdx.x;
-  const unsigned cb = (unsigned)(g % a.ncb);
-  g /= a.ncb;
-  const unsigned kk = (unsigned)(g % a.K);
-  const unsigned long long o = g / a.K + a.outer0;
-  const unsigned col = cb * C + c;
-  const bool active = col < a.ncol;
-  const unsigned ko = a.kbits ? brev_bits(kk, a.kbits) : kk;
-
-  const long long in_off = (long long)o * a.in_o + (long long)kk * a.in_k + (long long)col * a.in_c;
-  const long long out_off = (long long)o * a.out_o + (long long)ko * a.out_k + (long long)col * a.out_c;
-  const cpx<R>* __restrict__ in = reinterpret_cast<const cpx<R>*>(a.in) + in_off;
-  cpx<R>* __restrict__ out = reinterpret_cast<cpx<R>*>(a.out) + out_off;
-  const cpx<R>* __restrict__ tw = reinterpret_cast<const cpx<R>*>(a.tw);
-
-  cpx<R> v[PPT];
-  // ---- stage 0 inputs straight from HBM
-  {
-    constexpr int RAD = sched_radix(L, 0);
-    constexpr int NB = PPT / RAD;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int j = tau + b * TPL;
-#pragma unroll
-      for (int r = 0; r < RAD; ++r) {
-        const long long p = j + r * (L / RAD);
-        cpx<R> x = {R(0), R(0)};
-        if (active) {
-          x = in[p * a.in_p];
-          if (a.w_load) x = cscale(x, (R)a.w_load[(unsigned long long)(in_off + p * a.in_p) & a.w_mask]);
-        }
-        v[b * RAD + r] = a.inverse ? cswap(x) : x;
-      }
-    }
-  }
-
-  run_stages<R, L, MODE, 0>(v, lds, c, tau, tw);
-
-  // ---- last stage: point k = j + r*NS of the line is in v[b*RAD + r]
-  constexpr int RL = sched_radix(L, NP - 1);
-  constexpr int NSL = sched_ns(L, NP - 1);
-  constexpr int NBL = PPT / RL;
-  if constexpr (EPI == EPI_BITREV) {
-    // stage through LDS so the bit-reversed line is written with coalesced stores
-    if constexpr (NP > 1) __syncthreads();
-#pragma unroll
-    for (int b = 0; b < NBL; ++b) {
-      const int j = tau + b * TPL;
-#pragma unroll
-      for (int r = 0; r < RL; ++r) lds[G::idx(c, j + r * NSL)] = v[b * RL + r];
-    }
-    __syncthreads();
-    constexpr unsigned LG = ilog2c(L);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const int q = tau + m * TPL;
-      cpx<R> x = lds[G::idx(c, (int)brev_bits((unsigned)q, LG))];
-      if (a.inverse) x = cswap(x);
-      if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + (long long)q * a.out_p) & a.w_mask]);
-      if (active) out[(long long)q * a.out_p] = x;
-    }
-  } else {
-#pragma unroll
-    for (int b = 0; b < NBL; ++b) {
-      const int j = tau + b * TPL;
-#pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const int k = j + r * NSL;
-        cpx<R> x = v[b * RL + r];
-        if constexpr (EPI == EPI_TWIDDLE) {
-          // W_M^t, t = (sub-axis index of this column) * k mod M, from a two-level table
-          const unsigned t = ((col >> a.ptw_cshift) * (unsigned)k) & a.ptw_mmask;
-          const cpx<R> wl = reinterpret_cast<const cpx<R>*>(a.ptw_lo)[t & ((1u << a.ptw_hb) - 1u)];
-          const cpx<R> wh = reinterpret_cast<const cpx<R>*>(a.ptw_hi)[t >> a.ptw_hb];
-          x = cmul(x, cmul(wh, wl));
-        }
-        if (a.inverse) x = cswap(x);
-        if (active) {
-          if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + (long long)k * a.out_p) & a.w_mask]);
-          out[(long long)k * a.out_p] = x;
-        }
-      }
-    }
-  }
+  pass_block<R, L, MODE, EPI>(a, blockIdx.x, lds);
 }
 
 // ---- launch table ------------------------------------------------------------

@@ -41,11 +41,12 @@ constexpr int sched_twsize(int L) { return sched_twoff(L, sched_np(L)); }
 // ---- geometry -------------------------------------------------------------
 constexpr int geom_ppt(int L) { return L < 16 ? L : 16; }
 constexpr int geom_tpl(int L) { return L / geom_ppt(L); }
-// Lines per workgroup.  Contiguous lines: ~256 threads per workgroup.
-// Strided lines: enough adjacent columns for >= 128-byte row segments.
+// Lines per workgroup.  Contiguous lines: 256 threads per workgroup (one line
+// per workgroup from L = 4096 up).  Strided lines: at least 16 adjacent
+// columns (128-byte row segments) and 256 threads up to L = 256.
 constexpr int geom_cols(int L, int mode) {
   return mode == MODE_CONTIG ? (geom_tpl(L) >= 256 ? 1 : 256 / geom_tpl(L))
-                             : (L >= 256 ? 16 : (L == 128 ? 32 : 64));
+                             : (L >= 256 ? 16 : 256 / geom_tpl(L));
 }
 constexpr int geom_threads(int L, int mode) { return geom_cols(L, mode) * geom_tpl(L); }
 

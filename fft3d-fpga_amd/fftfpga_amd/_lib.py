@@ -15,6 +15,17 @@ _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libfftfpga.so")
 
 
+class XForm3D(ctypes.Structure):
+    """fftamd_xform3d: one transform of a multi-transform plan (fftfpga_hip.h)."""
+
+    _fields_ = [
+        ("n", ctypes.c_uint * 3),
+        ("reserved", ctypes.c_uint),
+        ("in_offset", ctypes.c_ulonglong),
+        ("out_offset", ctypes.c_ulonglong),
+    ]
+
+
 class FpgaT(ctypes.Structure):
     """fpga_t: per-call timings in ms plus the validity flag (fftfpga.h:32-39)."""
 
@@ -60,6 +71,18 @@ SIGNATURES = {
     "fftamd_z2z_dev": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _u, _vp]),
     "fftamd_c2c_weighted_dev": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _vp, _vp, _vp]),
     "fftfpgaf_c2c_3d_ddr_dev": (_i, [_u, _vp, _vp, _b, _u, _vp]),
+    "fftamd_multi_create": (_i, [ctypes.POINTER(_vp), _u, _vp, _vp, _vp, _i]),
+    "fftamd_multi_exec": (_i, [_vp, _vp]),
+    "fftamd_multi_check": (_i, [_vp, _vp]),
+    "fftamd_multi_tune": (_i, [_vp, _vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_double)]),
+    "fftamd_multi_schedule": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_u), ctypes.POINTER(_u),
+                                   ctypes.POINTER(_u), ctypes.POINTER(_u)]),
+    "fftamd_multi_set_schedule": (_i, [_vp, _i, _u, _u]),
+    "fftamd_multi_destroy": (None, [_vp]),
+    "fftamd_multi_set_spin_limit": (None, [_u]),
+    "fftamd_graph_capture": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i, _vp]),
+    "fftamd_graph_launch": (_i, [_vp, _vp]),
+    "fftamd_graph_destroy": (None, [_vp]),
     "fftamd_last_error": (ctypes.c_char_p, []),
     "fftamd_plan_cache_clear": (None, []),
     "fftamd_plan_cache_size": (ctypes.c_size_t, []),

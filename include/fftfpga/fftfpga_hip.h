@@ -51,6 +51,53 @@ int fftamd_c2c_weighted_dev(int rank, const unsigned *dims, unsigned batch, cons
 int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2 *d_in, float2 *d_out, bool inv,
                             unsigned how_many, void *stream);
 
+/* ---- transforms of uneven sizes in one persistent launch --------------------------
+ *
+ * A batch of 3D transforms of different power-of-two shapes (each axis <= 256)
+ * placed anywhere in two device buffers.  The reference's batched call takes one
+ * N per call (api/src/fft3d.c:396-913); this plan runs every pass of every
+ * transform in a single persistent kernel: per phase (axis), 256-thread
+ * workgroups take chunks of tiles from a device-side work queue, largest
+ * transforms first, with a grid barrier between phases.  Schedules (single
+ * launch or one launch per phase, workgroups per CU, tiles per queue ticket)
+ * are picked by fftamd_multi_tune and cached per shape signature. */
+typedef struct fftamd_multi_s *fftamd_multi_t;
+typedef struct fftamd_graph_s *fftamd_graph_t;
+
+typedef struct {
+  unsigned n[3];                 /* dims (slowest .. fastest), x = n[2] contiguous */
+  unsigned reserved;             /* must be 0 */
+  unsigned long long in_offset;  /* first element, in complex elements from d_in */
+  unsigned long long out_offset; /* first element, in complex elements from d_out */
+} fftamd_xform3d;
+
+int fftamd_multi_create(fftamd_multi_t *plan, unsigned count, const fftamd_xform3d *xf,
+                        const float2 *d_in, float2 *d_out, int inverse);
+/* Enqueue the plan on a stream (asynchronous). */
+int fftamd_multi_exec(fftamd_multi_t plan, void *stream);
+/* Synchronise the stream and report whether the last launch's grid barrier
+ * wait exceeded its bound (-1 with a message) or not (0). */
+int fftamd_multi_check(fftamd_multi_t plan, void *stream);
+/* Time every schedule candidate on the plan's buffers (reps launches each),
+ * keep the fastest and append it to cache_path (NULL: no cache).  Returns 0
+ * after tuning, 1 when the schedule came from the cache, -1 on error. */
+int fftamd_multi_tune(fftamd_multi_t plan, void *stream, const char *cache_path, int reps,
+                      double *best_us);
+/* mode 0: one launch with grid barriers; mode 1: one launch per phase. */
+int fftamd_multi_schedule(fftamd_multi_t plan, int *mode, unsigned *grid, unsigned *chunk,
+                          unsigned *items, unsigned *phases);
+int fftamd_multi_set_schedule(fftamd_multi_t plan, int mode, unsigned grid, unsigned chunk);
+void fftamd_multi_destroy(fftamd_multi_t plan);
+/* Bound on the grid barrier's polls (default 2^22); a tiny value forces the
+ * timeout path so callers can test their error handling. */
+void fftamd_multi_set_spin_limit(unsigned polls);
+
+/* Capture plans[0..n-1] executed in order (e.g. a forward/inverse pair) into a
+ * HIP graph on a non-null stream; replay with fftamd_graph_launch. */
+int fftamd_graph_capture(fftamd_graph_t *graph, fftamd_multi_t *plans, int n, void *stream);
+int fftamd_graph_launch(fftamd_graph_t graph, void *stream);
+void fftamd_graph_destroy(fftamd_graph_t graph);
+
 /* Last error message of the calling thread ("" if none). */
 const char *fftamd_last_error(void);
 

@@ -1,0 +1,171 @@
+"""Transforms of uneven sizes in one persistent launch (fftamd_multi_*).
+
+Bars: every transform of a mixed batch within rel-L2 2e-6 of the FP64
+oracle; the multi-transform kernel runs the same tile code as the
+per-transform path, so on the 448-transform workload its output must equal
+the per-transform output bit for bit; HIP-graph replays must equal the eager
+result on every replay; a grid-barrier wait forced past its bound from the
+host must be reported, not hang.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+TOL = 2e-6
+
+
+@pytest.fixture(autouse=True)
+def announce(request):
+    sys.__stderr__.write(f"\n[gpu-case] {request.node.nodeid}\n")
+    sys.__stderr__.flush()
+
+
+def _pack(shapes, seed):
+    import torch
+    from fftfpga_amd import datagen
+    from fftfpga_amd.multi import packed_offsets
+    offs, total = packed_offsets(shapes)
+    host = np.empty(total, np.complex64)
+    for i, (s, o) in enumerate(zip(shapes, offs)):
+        n = int(np.prod(s))
+        host[o:o + n] = datagen.gauss(n, datagen.member_seed(seed, i))
+    return host, torch.from_numpy(host).cuda(), offs
+
+
+MIXED = [(16, 16, 16), (64, 64, 64), (8, 8, 8), (128, 128, 128), (32, 32, 32), (4, 8, 16),
+         (64, 64, 64), (2, 2, 2), (32, 32, 32), (256, 16, 8), (1, 64, 32), (16, 16, 16)]
+
+
+@pytest.mark.parametrize("mode,per_cu,chunk", [(0, None, 4), (1, None, 1), (0, 1, 16), (1, 2, 3)])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_mixed_batch_vs_oracle(gpu_lib, orc, mode, per_cu, chunk, inverse):
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    host, d_in, offs = _pack(MIXED, 5)
+    d_out = torch.zeros_like(d_in)
+    plan = MultiPlan(MIXED, d_in, d_out, inverse=inverse)
+    sch = plan.schedule()
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    grid = sch["grid"] if per_cu is None else min(sch["grid"], per_cu * ncu)
+    plan.set_schedule(mode, grid, chunk)
+    plan()
+    plan.check()
+    out = d_out.cpu().numpy()
+    for s, o in zip(MIXED, offs):
+        n = int(np.prod(s))
+        ref = orc.fftnd(host[o:o + n], list(s), inverse=inverse)
+        err = orc.rel_l2(out[o:o + n], ref)
+        assert err <= TOL, (s, err)
+    plan.close()
+
+
+def test_mixed_batch_in_place(gpu_lib, orc):
+    from fftfpga_amd.multi import MultiPlan
+    host, d, offs = _pack(MIXED, 6)
+    plan = MultiPlan(MIXED, d, d)
+    plan()
+    plan.check()
+    out = d.cpu().numpy()
+    for s, o in zip(MIXED, offs):
+        n = int(np.prod(s))
+        assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s))) <= TOL
+
+
+def _config3_shapes():
+    from fftfpga_amd import datagen
+    sizes = datagen.batch_order({32: 256, 64: 160, 128: 32}, 2)
+    return [(n, n, n) for n in sizes]
+
+
+def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc):
+    import torch
+    from fftfpga_amd import c2c_dev
+    from fftfpga_amd.multi import MultiPlan
+    shapes = _config3_shapes()
+    assert len(shapes) == 448
+    host, d_in, offs = _pack(shapes, 2)
+    d_out = torch.empty_like(d_in)
+    plan = MultiPlan(shapes, d_in, d_out)
+    plan()
+    plan.check()
+    ref = torch.empty_like(d_in)
+    for s, o in zip(shapes, offs):
+        n = int(np.prod(s))
+        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s))
+    torch.cuda.synchronize()
+    assert torch.equal(d_out, ref)
+    out = d_out.cpu().numpy()
+    for i in (0, 1, 2, 100, 447):
+        s, o = shapes[i], offs[i]
+        n = int(np.prod(s))
+        assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s))) <= TOL
+    plan.close()
+
+
+def test_graph_forward_inverse_pair_replays(gpu_lib):
+    import torch
+    from fftfpga_amd.multi import Graph, MultiPlan
+    shapes = [(32, 32, 32)] * 8 + [(64, 64, 64)] * 4 + [(128, 128, 128)] * 2
+    host, d_in, offs = _pack(shapes, 9)
+    mid = torch.empty_like(d_in)
+    back = torch.empty_like(d_in)
+    fwd = MultiPlan(shapes, d_in, mid)
+    inv = MultiPlan(shapes, mid, back, inverse=True)
+    fwd()
+    inv()
+    fwd.check()
+    eager = back.clone()
+    g = Graph([fwd, inv])
+    for r in range(3):
+        back.zero_()
+        mid.mul_(0)
+        d_in.add_(0)  # eager work between replays
+        torch.cuda.synchronize()
+        g()
+        torch.cuda.synchronize()
+        assert torch.equal(back, eager), f"replay {r} differs from the eager result"
+    # round trip returns N^3 x
+    out = back.cpu().numpy()
+    for s, o in zip(shapes, offs):
+        n = int(np.prod(s))
+        err = np.linalg.norm(out[o:o + n] / n - host[o:o + n]) / np.linalg.norm(host[o:o + n])
+        assert err <= TOL
+    g.close()
+
+
+def test_barrier_bound_is_reported_not_hung(gpu_lib):
+    import torch
+    from fftfpga_amd.multi import MultiPlan, set_spin_limit
+    shapes = [(64, 64, 64)] * 16
+    _, d_in, _ = _pack(shapes, 3)
+    d_out = torch.empty_like(d_in)
+    plan = MultiPlan(shapes, d_in, d_out)
+    plan.set_schedule(0, plan.schedule()["grid"], 1)
+    set_spin_limit(1)  # the host forces the bounded wait to expire
+    try:
+        plan()
+        with pytest.raises(RuntimeError, match="bound"):
+            plan.check()
+    finally:
+        set_spin_limit(1 << 22)
+    plan()
+    plan.check()  # a normal launch afterwards is fine
+
+
+def test_autotuner_picks_and_caches(gpu_lib, tmp_path):
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    shapes = [(32, 32, 32)] * 32 + [(64, 64, 64)] * 8
+    _, d_in, _ = _pack(shapes, 4)
+    d_out = torch.empty_like(d_in)
+    cache = str(tmp_path / "tune.txt")
+    plan = MultiPlan(shapes, d_in, d_out)
+    r1 = plan.tune(cache, reps=2)
+    assert not r1["from_cache"] and r1["best_us"] > 0 and r1["mode"] in (0, 1)
+    plan2 = MultiPlan(shapes, d_in, d_out)
+    r2 = plan2.tune(cache, reps=2)
+    assert r2["from_cache"] and (r2["mode"], r2["grid"], r2["chunk"]) == (r1["mode"], r1["grid"], r1["chunk"])
+    plan2()
+    plan2.check()
