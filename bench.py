@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the torch.fft comparison")
     ap.add_argument("--no-kernel-events", action="store_true",
                     help="time the steps without per-kernel HIP events (diagnostic)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 10 plain device copies of one workload-sized array (counter calibration)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the uneven-size batched vs plain per-transform comparison")
     return ap.parse_args()
@@ -163,6 +165,15 @@ def main():
     rank = len(dims)
     npts = math.prod(dims) * batch
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        a = torch.ones(npts, dtype=torch.complex64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": 8 * npts, "bytes_written": 8 * npts,
+                          "launches": 10}))
+        return
     x = torch.from_numpy(datagen.uniform(npts, seed)).to(dev).reshape(batch, *dims)
     X = torch.empty_like(x)
     y = torch.empty_like(x)
