@@ -72,6 +72,7 @@ struct PassDesc {
 
 struct Plan {
   int prec = PREC_F32;
+  bool split = false;  // radix-16 stages on FP16 matrix cores (hi+lo operands)
   std::vector<PassDesc> passes;
   size_t total = 0;  // points including batch
   size_t per = 0;    // points of one transform
@@ -263,6 +264,11 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
                       bool aliased, Plan& plan) {
   plan = Plan();
   plan.prec = prec;
+  plan.split = (flags & FLAG_SPLIT_MFMA) != 0;
+  if (plan.split && prec != PREC_F32) {
+    set_error("split-precision mode is defined for single precision only");
+    return -1;
+  }
   size_t tot = 1;
   for (int d = 0; d < rank; ++d) tot *= dims[d];
   plan.total = tot * batch;
@@ -433,8 +439,8 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
 static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
                     Plan** out) {
   State& st = S();
-  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & FLAG_BITREV_1D, prec,
-              aliased);
+  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & (FLAG_BITREV_1D | FLAG_SPLIT_MFMA),
+              prec, aliased);
   auto it = st.plans.find(key);
   if (it == st.plans.end()) {
     Plan p;
@@ -501,11 +507,14 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       Prof& pf = P();
       const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
-      int rc = launch_fft_pass(plan.prec, p.lg, p.mode, p.epi, a, per_outer * no, stream);
+      // split mode applies to lines with a radix-16 stage (16..4096 points)
+      const bool split = plan.split && L >= 16 && L <= 4096;
+      int rc = launch_fft_pass(plan.prec, p.lg, p.mode, p.epi, a, per_outer * no, stream, split);
       if (prof) {
         HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
         char lab[96];
-        snprintf(lab, sizeof(lab), "fft_pass_%s_L%d%s%s", plan.prec == PREC_F32 ? "f32" : "f64", L,
+        snprintf(lab, sizeof(lab), "fft_pass_%s%s_L%d%s%s", plan.prec == PREC_F32 ? "f32" : "f64",
+                 split ? "_split" : "", L,
                  p.mode == MODE_CONTIG ? "_contig" : "_strided",
                  p.epi == EPI_TWIDDLE ? "_twiddle" : (p.epi == EPI_BITREV ? "_bitrev" : ""));
         pf.recs.push_back({lab, pf.pool[pf.used], pf.pool[pf.used + 1]});

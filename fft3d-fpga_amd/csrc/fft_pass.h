@@ -33,7 +33,8 @@ struct PassArgs {
 };
 
 // Launches one pass; returns 0, -1 for an unsupported (prec, lg, mode, epi), -2 on a launch error.
+// split: radix-16 stages on FP16 matrix cores with hi+lo operands (FP32 only).
 int launch_fft_pass(int prec, int lg, int mode, int epi, const PassArgs& a, unsigned long long blocks,
-                    void* stream);
+                    void* stream, bool split = false);
 
 }  // namespace fftamd

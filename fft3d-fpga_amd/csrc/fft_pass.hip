@@ -17,25 +17,25 @@
 
 namespace fftamd {
 
-template <typename R, int L, int MODE, int EPI>
+template <typename R, int L, int MODE, int EPI, bool SPLIT>
 __global__ void __launch_bounds__(geom_threads(L, MODE))
     k_fft_pass(const PassArgs a) {
   using G = Geom<L, MODE>;
-  constexpr bool NEED_LDS = G::NP > 1 || EPI == EPI_BITREV;
-  __shared__ cpx<R> lds[NEED_LDS ? G::LDS_ELEMS : 1];
-  pass_block<R, L, MODE, EPI>(a, blockIdx.x, lds);
+  constexpr bool NEED_LDS = G::NP > 1 || EPI == EPI_BITREV || SPLIT;
+  __shared__ cpx<R> lds[NEED_LDS ? (SPLIT ? G::LDS_ELEMS_SPLIT : G::LDS_ELEMS) : 1];
+  pass_block<R, L, MODE, EPI, SPLIT>(a, blockIdx.x, lds);
 }
 
 // ---- launch table ------------------------------------------------------------
 
-template <typename R, int MODE, int EPI, int LG>
+template <typename R, int MODE, int EPI, int LG, bool SPLIT>
 static void launch_one(const PassArgs& a, unsigned long long blocks, hipStream_t s) {
   constexpr int L = 1 << LG;
-  hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI>), dim3((unsigned)blocks), dim3(geom_threads(L, MODE)),
-                     0, s, a);
+  hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI, SPLIT>), dim3((unsigned)blocks),
+                     dim3(geom_threads(L, MODE)), 0, s, a);
 }
 
-template <typename R, int MODE, int EPI, int LG>
+template <typename R, int MODE, int EPI, int LG, bool SPLIT>
 static bool dispatch_lg(int lg, const PassArgs& a, unsigned long long blocks, hipStream_t s) {
   if constexpr (LG > 13) {
     return false;
@@ -43,38 +43,45 @@ static bool dispatch_lg(int lg, const PassArgs& a, unsigned long long blocks, hi
     if (lg == LG) {
       constexpr int L = 1 << LG;
       constexpr int prec = sizeof(R) == 4 ? PREC_F32 : PREC_F64;
-      constexpr bool ok = (MODE == MODE_CONTIG ? L <= lmax_contig(prec) : L <= lmax_strided(prec)) &&
+      // split mode exists for FP32 lines with at least one radix-16 stage
+      constexpr bool split_ok = !SPLIT || (sizeof(R) == 4 && L >= 16 && L <= 4096);
+      constexpr bool ok = split_ok &&
+                          (MODE == MODE_CONTIG ? L <= lmax_contig(prec) : L <= lmax_strided(prec)) &&
                           !(MODE == MODE_CONTIG && EPI == EPI_TWIDDLE) &&
                           !(MODE == MODE_STRIDED && EPI == EPI_BITREV);
       if constexpr (ok) {
-        launch_one<R, MODE, EPI, LG>(a, blocks, s);
+        launch_one<R, MODE, EPI, LG, SPLIT>(a, blocks, s);
         return true;
       } else {
         return false;
       }
     }
-    return dispatch_lg<R, MODE, EPI, LG + 1>(lg, a, blocks, s);
+    return dispatch_lg<R, MODE, EPI, LG + 1, SPLIT>(lg, a, blocks, s);
   }
 }
 
-template <typename R>
+template <typename R, bool SPLIT>
 static bool dispatch_prec(int lg, int mode, int epi, const PassArgs& a, unsigned long long blocks,
                           hipStream_t s) {
   if (mode == MODE_CONTIG) {
-    if (epi == EPI_NONE) return dispatch_lg<R, MODE_CONTIG, EPI_NONE, 1>(lg, a, blocks, s);
-    if (epi == EPI_BITREV) return dispatch_lg<R, MODE_CONTIG, EPI_BITREV, 1>(lg, a, blocks, s);
+    if (epi == EPI_NONE) return dispatch_lg<R, MODE_CONTIG, EPI_NONE, 1, SPLIT>(lg, a, blocks, s);
+    if (epi == EPI_BITREV) return dispatch_lg<R, MODE_CONTIG, EPI_BITREV, 1, SPLIT>(lg, a, blocks, s);
   } else {
-    if (epi == EPI_NONE) return dispatch_lg<R, MODE_STRIDED, EPI_NONE, 1>(lg, a, blocks, s);
-    if (epi == EPI_TWIDDLE) return dispatch_lg<R, MODE_STRIDED, EPI_TWIDDLE, 1>(lg, a, blocks, s);
+    if (epi == EPI_NONE) return dispatch_lg<R, MODE_STRIDED, EPI_NONE, 1, SPLIT>(lg, a, blocks, s);
+    if (epi == EPI_TWIDDLE) return dispatch_lg<R, MODE_STRIDED, EPI_TWIDDLE, 1, SPLIT>(lg, a, blocks, s);
   }
   return false;
 }
 
 int launch_fft_pass(int prec, int lg, int mode, int epi, const PassArgs& a, unsigned long long blocks,
-                    void* stream) {
+                    void* stream, bool split) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  bool ok = prec == PREC_F32 ? dispatch_prec<float>(lg, mode, epi, a, blocks, s)
-                             : dispatch_prec<double>(lg, mode, epi, a, blocks, s);
+  bool ok;
+  if (prec == PREC_F32)
+    ok = split ? dispatch_prec<float, true>(lg, mode, epi, a, blocks, s)
+               : dispatch_prec<float, false>(lg, mode, epi, a, blocks, s);
+  else
+    ok = !split && dispatch_prec<double, false>(lg, mode, epi, a, blocks, s);
   if (!ok) return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

@@ -17,6 +17,7 @@
 #include "fft_device.h"
 #include "fft_pass.h"
 #include "fft_schedule.h"
+#include "fft_split.h"
 
 namespace fftamd {
 
@@ -30,14 +31,18 @@ struct Geom {
   // contiguous: [line][p + p/16]; strided: [p][col] with one pad column
   static constexpr int LP = L + L / 16;
   static constexpr int LDS_ELEMS = MODE == MODE_CONTIG ? C * LP : L * (C + 1);
+  // split-precision radix-16 stages borrow the buffer as per-wave scratch
+  static constexpr int SPLIT_ELEMS = ((NT + 63) / 64) * SPLIT_WAVE_FLOATS / 2;
+  static constexpr int LDS_ELEMS_SPLIT = LDS_ELEMS > SPLIT_ELEMS ? LDS_ELEMS : SPLIT_ELEMS;
   static __device__ __forceinline__ int idx(int c, int p) {
     if constexpr (MODE == MODE_CONTIG) return c * LP + p + (p >> 4);
     else return p * (C + 1) + c;
   }
 };
 
-template <typename R, int L, int MODE, int S>
-__device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* __restrict__ tw) {
+template <typename R, int L, int MODE, int S, bool SPLIT = false>
+__device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* __restrict__ tw,
+                                              cpx<R>* lds = nullptr) {
   using G = Geom<L, MODE>;
   constexpr int RAD = sched_radix(L, S);
   constexpr int NS = sched_ns(L, S);
@@ -53,8 +58,16 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* 
       for (int r = 1; r < RAD; ++r) v[b * RAD + r] = cmul(v[b * RAD + r], t[r - 1]);
     }
   }
+  if constexpr (SPLIT && RAD == 16 && sizeof(R) == 4) {
+    static_assert(NB == 1, "split radix-16 stages hold one butterfly per lane");
+    __syncthreads();  // every wave finished reading the stage buffer it is about to borrow
+    float* scratch = reinterpret_cast<float*>(lds) + (threadIdx.x >> 6) * SPLIT_WAVE_FLOATS;
+    dft16_split_mfma(reinterpret_cast<cpx<float>*>(v), scratch);
+    __syncthreads();  // scratch is released before the stage buffer is written again
+  } else {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) dft<R, RAD>(v + b * RAD);
+    for (int b = 0; b < NB; ++b) dft<R, RAD>(v + b * RAD);
+  }
 }
 
 // write stage-S results to LDS at their Stockham destinations
@@ -87,7 +100,7 @@ __device__ __forceinline__ void stage_load_lds(cpx<R>* v, const cpx<R>* lds, int
   }
 }
 
-template <typename R, int L, int MODE, int S>
+template <typename R, int L, int MODE, int S, bool SPLIT = false>
 __device__ __forceinline__ void run_stages(cpx<R>* v, cpx<R>* lds, int c, int tau,
                                            const cpx<R>* __restrict__ tw) {
   using G = Geom<L, MODE>;
@@ -96,11 +109,12 @@ __device__ __forceinline__ void run_stages(cpx<R>* v, cpx<R>* lds, int c, int ta
       __syncthreads();  // previous stage's LDS writes are visible
       stage_load_lds<R, L, MODE, S>(v, lds, c, tau);
     }
-    stage_compute<R, L, MODE, S>(v, tau, tw);
+    stage_compute<R, L, MODE, S, SPLIT>(v, tau, tw, lds);
     if constexpr (S + 1 < G::NP) {
-      if constexpr (S > 0) __syncthreads();  // every thread finished reading this stage
+      constexpr bool split_stage = SPLIT && sched_radix(L, S) == 16;
+      if constexpr (S > 0 && !split_stage) __syncthreads();  // every thread finished reading this stage
       stage_store_lds<R, L, MODE, S>(v, lds, c, tau);
-      run_stages<R, L, MODE, S + 1>(v, lds, c, tau, tw);
+      run_stages<R, L, MODE, S + 1, SPLIT>(v, lds, c, tau, tw);
     }
   }
 }
@@ -108,7 +122,7 @@ __device__ __forceinline__ void run_stages(cpx<R>* v, cpx<R>* lds, int c, int ta
 // One workgroup tile of a pass: block index g (decoded into column block,
 // middle and outer line index).  Used by k_fft_pass (one tile per workgroup)
 // and by the persistent multi-transform kernel (many tiles per workgroup).
-template <typename R, int L, int MODE, int EPI>
+template <typename R, int L, int MODE, int EPI, bool SPLIT = false>
 __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds) {
   using G = Geom<L, MODE>;
   constexpr int PPT = G::PPT, TPL = G::TPL, C = G::C;
@@ -153,7 +167,7 @@ __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long
     }
   }
 
-  run_stages<R, L, MODE, 0>(v, lds, c, tau, tw);
+  run_stages<R, L, MODE, 0, SPLIT>(v, lds, c, tau, tw);
 
   // ---- last stage: point k = j + r*NS of the line is in v[b*RAD + r]
   constexpr int RL = sched_radix(L, NP - 1);
@@ -161,7 +175,7 @@ __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long
   constexpr int NBL = PPT / RL;
   if constexpr (EPI == EPI_BITREV) {
     // stage through LDS so the bit-reversed line is written with coalesced stores
-    if constexpr (NP > 1) __syncthreads();
+    if constexpr (NP > 1 || SPLIT) __syncthreads();
 #pragma unroll
     for (int b = 0; b < NBL; ++b) {
       const int j = tau + b * TPL;

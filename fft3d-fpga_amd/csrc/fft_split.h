@@ -1,0 +1,145 @@
+// fft_split.h -- split-precision radix-16 butterflies on gfx950 FP16 matrix
+// cores (v_mfma_f32_16x16x32_f16), FP32 accumulation.
+//
+// A radix-16 stage of a wave is 64 independent 16-point DFTs (one per lane).
+// They are computed 16 at a time as real matrix products
+//   [Yr] = [ C  S] [Xr]      C[i][t] = cos(2 pi i t / 16), S = sin(...)
+//   [Yi]   [-S  C] [Xi]
+// i.e. D(16x16) = A(16x32) B(32x16) with B holding 16 vectors as columns.
+// FP16 carries 11 significant bits, so every operand is split as
+// x = hi + lo (hi = fp16(x), lo = fp16(x - hi)) and the product is
+// A_hi B_hi + A_hi B_lo + A_lo B_hi (the dropped A_lo B_lo term is ~2^-23
+// relative): 6 MFMAs per 16 vectors.  Before splitting, each group of 16
+// vectors is scaled by a power of two so its largest component lies in
+// [2^14, 2^15): FP16 cannot overflow, and the scale is exact.
+//
+// Operand layout of v_mfma_f32_16x16x32_f16 (checked on the device with an
+// integer-valued probe): lane l holds A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16]
+// for j = 0..7, and receives D[4(l/16)+e][l%16] for e = 0..3.
+//
+// Data moves between the "one vector per lane" form used by the radix stages
+// and the operand form through a per-wave LDS scratch of 64 rows x 36 floats
+// (32 values + 4 pad floats so the 16 lanes reading one operand slice hit
+// distinct banks).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_device.h"
+
+namespace fftamd {
+
+typedef _Float16 split_half8 __attribute__((ext_vector_type(8)));
+typedef float split_float4 __attribute__((ext_vector_type(4)));
+
+constexpr int SPLIT_ROW = 36;                      // floats per scratch row
+constexpr int SPLIT_WAVE_FLOATS = 64 * SPLIT_ROW;  // per wave
+
+// cos(2 pi m / 16) for m mod 16, from the exact constants of the radix-16 butterfly
+__device__ __forceinline__ float cos16(int m) {
+  m &= 15;
+  const float c1 = K<float>::C16, c2 = K<float>::C8, c3 = K<float>::S16;
+  const float t[5] = {1.0f, c1, c2, c3, 0.0f};
+  // cos is even and cos(pi - x) = -cos(x)
+  int a = m <= 8 ? m : 16 - m;  // 0..8
+  return a <= 4 ? t[a] : -t[8 - a];
+}
+
+__device__ __forceinline__ void split2(float x, _Float16* hi, _Float16* lo) {
+  const _Float16 h = (_Float16)x;
+  *hi = h;
+  *lo = (_Float16)(x - (float)h);
+}
+
+// Radix-16 DFT (forward sign) of the 16 points v[0..15] held by every lane of
+// the wave, in place.  All 64 lanes must call it together; `scratch` is this
+// wave's private LDS region of SPLIT_WAVE_FLOATS floats.
+__device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) {
+  const int lane = threadIdx.x & 63;
+  const int row = lane & 15, part = lane >> 4;
+  // ---- constant DFT operands A1 = [C | S], A2 = [-S | C], split hi/lo
+  split_half8 a1h, a1l, a2h, a2l;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int t = 8 * (part & 1) + j;  // point index inside the re (part 0,1) or im (2,3) half
+    const int m = row * t;
+    const float c = cos16(m), s = cos16(m - 4);  // sin(x) = cos(x - pi/2)
+    const float e1 = part < 2 ? c : s;
+    const float e2 = part < 2 ? -s : c;
+    _Float16 h, l;
+    split2(e1, &h, &l);
+    a1h[j] = h;
+    a1l[j] = l;
+    split2(e2, &h, &l);
+    a2h[j] = h;
+    a2l[j] = l;
+  }
+  // ---- this lane's vector -> scratch row `lane`
+  float* my = scratch + lane * SPLIT_ROW;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    my[t] = v[t].x;
+    my[16 + t] = v[t].y;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  split_float4 yr[4], yi[4];
+  float scl[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    // operand slice: vector (16g + row), values 8*part .. 8*part+7 of [re 0..15 | im 0..15]
+    const float* src = scratch + (16 * g + row) * SPLIT_ROW + 8 * part;
+    float x[8];
+    float mx = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      x[j] = src[j];
+      mx = fmaxf(mx, fabsf(x[j]));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    // power-of-two scale bringing the group maximum into [2^14, 2^15)
+    int ex = 0;
+    (void)frexpf(mx, &ex);  // mx = f * 2^ex, f in [0.5, 1)
+    const int sh = mx > 0.0f ? 15 - ex : 0;
+    const float s = ldexpf(1.0f, sh);
+    scl[g] = ldexpf(1.0f, -sh);
+    split_half8 bh, bl;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      _Float16 h, l;
+      split2(x[j] * s, &h, &l);
+      bh[j] = h;
+      bl[j] = l;
+    }
+    split_float4 z = {0.f, 0.f, 0.f, 0.f};
+    split_float4 r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bh, z, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bl, r, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, bh, r, 0, 0, 0);
+    split_float4 q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh, z, 0, 0, 0);
+    q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl, q, 0, 0, 0);
+    q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh, q, 0, 0, 0);
+    yr[g] = r;
+    yi[g] = q;
+  }
+  // ---- results back to one vector per lane: lane holds points 4*part+e of vector 16g+row
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    float* dst = scratch + (16 * g + row) * SPLIT_ROW;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      dst[4 * part + e] = yr[g][e] * scl[g];
+      dst[16 + 4 * part + e] = yi[g][e] * scl[g];
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int t = 0; t < 16; ++t) v[t] = {my[t], my[16 + t]};
+}
+
+}  // namespace fftamd

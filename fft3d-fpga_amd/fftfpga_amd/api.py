@@ -19,6 +19,7 @@ import numpy as np
 from ._lib import FpgaT, last_error, load
 
 FFTAMD_BITREV_1D = 0x1
+FFTAMD_SPLIT_MFMA = 0x2
 
 
 def _ptr(a, dtype, need: int):
@@ -137,9 +138,11 @@ def _stream_handle(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
-def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=None):
+def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=None,
+            split: bool = False):
     """Transform the last `rank` dims of a contiguous complex64/complex128 GPU
-    tensor; leading dims are the batch.  Returns `out` (allocated if None)."""
+    tensor; leading dims are the batch.  Returns `out` (allocated if None).
+    split=True selects the split-precision FP16 matrix-core radix-16 stages."""
     import torch
     if not x.is_cuda:
         raise ValueError("c2c_dev needs a GPU tensor")
@@ -161,8 +164,9 @@ def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=No
         return out
     cd = (ctypes.c_uint * rank)(*dims)
     fn = load().fftamd_c2c_dev if x.dtype == torch.complex64 else load().fftamd_z2z_dev
+    flags = (FFTAMD_BITREV_1D if bitrev_1d else 0) | (FFTAMD_SPLIT_MFMA if split else 0)
     rc = fn(rank, cd, batch, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-            1 if inverse else 0, FFTAMD_BITREV_1D if bitrev_1d else 0, _stream_handle(x))
+            1 if inverse else 0, flags, _stream_handle(x))
     if rc != 0:
         raise RuntimeError(f"fftamd transform failed: {last_error()}")
     return out

@@ -24,6 +24,11 @@ extern "C" {
 
 /* flags */
 #define FFTAMD_BITREV_1D 0x1u  /* rank 1: leave each transform bit-reversed, as fftfpgaf_c2c_1d */
+/* Split precision: radix-16 stages as 16x16 complex DFT products on FP16 matrix
+ * cores (v_mfma_f32_16x16x32_f16) with hi+lo split operands, power-of-two block
+ * scaling and FP32 accumulation; other stages and twiddles stay FP32.  Single
+ * precision only; lines of 16..4096 points use it. */
+#define FFTAMD_SPLIT_MFMA 0x2u
 
 /* Single precision, rank 1..3, every dimension a power of two, `batch`
  * contiguous transforms.  inverse != 0: unnormalised backward transform. */
