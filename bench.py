@@ -131,6 +131,14 @@ def batched_vs_plain(steps, warmup):
     ms_eager = run(plan)
     ms_plain = run(plain)
     plan.check()
+    # the same batch with split-precision (FP16 matrix-core) radix-16 stages
+    d_split = torch.empty_like(d_in)
+    splan = MultiPlan(shapes, d_in, d_split, split=True)
+    stune = splan.tune(reps=3)
+    sgraph = Graph([splan])
+    ms_split = run(sgraph)
+    splan.check()
+    split_rel = (torch.linalg.vector_norm(d_split - d_out) / torch.linalg.vector_norm(d_out)).item()
     flops = sum(flops_per_transform(sh) for sh in shapes)
     alg = 16.0 * total
     same = bool(torch.equal(d_out, d_ref))
@@ -147,6 +155,10 @@ def batched_vs_plain(steps, warmup):
         "roofline_frac_batched": round(alg / (ms_graph * 1e-3) / HBM_PEAK, 4),
         "schedule": {k: tune[k] for k in ("mode", "grid", "chunk", "items", "phases", "from_cache")},
         "outputs_identical_to_plain": same,
+        "split_precision": {"ms_batched_graph": round(ms_split, 4),
+                            "speedup_vs_plain": round(ms_plain / ms_split, 2),
+                            "rel_l2_vs_fp32_path": split_rel,
+                            "schedule": {k: stune[k] for k in ("mode", "grid", "chunk")}},
     }
 
 

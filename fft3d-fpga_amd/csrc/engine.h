@@ -50,7 +50,8 @@ int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din
 
 // ---- multi-transform plans (uneven sizes, one persistent launch) ---------------
 int multi_create(void** handle, unsigned count, const unsigned* dims3, const unsigned long long* in_off,
-                 const unsigned long long* out_off, const void* din, void* dout, bool inverse);
+                 const unsigned long long* out_off, const void* din, void* dout, bool inverse,
+                 unsigned flags);
 int multi_exec(void* handle, void* stream);
 int multi_check(void* handle, void* stream);
 void multi_destroy(void* handle);

@@ -553,6 +553,7 @@ struct MultiPlan {
   MultiCtl* d_ctl = nullptr;
   unsigned grid = 0, chunk = 4;
   int mode = 0;  // 0: one launch, grid barriers between phases; 1: one launch per phase
+  bool split = false;
   std::string sig;
   size_t points = 0;
   ~MultiPlan() {
@@ -565,10 +566,10 @@ struct MultiPlan {
 
 static unsigned g_multi_spin_limit = 1u << 22;
 
-static int multi_default_grid(unsigned* grid, unsigned* per_cu_max) {
+static int multi_default_grid(unsigned* grid, unsigned* per_cu_max, bool split) {
   int ncu = 0, per = 0;
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, S().dev < 0 ? 0 : S().dev));
-  if (multi_max_blocks_per_cu(&per) || per <= 0) {
+  if (multi_max_blocks_per_cu(&per, split) || per <= 0) {
     set_error("occupancy query for the multi-transform kernel failed");
     return -1;
   }
@@ -578,7 +579,8 @@ static int multi_default_grid(unsigned* grid, unsigned* per_cu_max) {
 }
 
 int multi_create(void** handle, unsigned count, const unsigned* dims3, const unsigned long long* in_off,
-                 const unsigned long long* out_off, const void* din, void* dout, bool inverse) {
+                 const unsigned long long* out_off, const void* din, void* dout, bool inverse,
+                 unsigned flags) {
   State& st = S();
   std::lock_guard<std::recursive_mutex> lk(st.mu);
   if (runtime_ensure_device()) return -1;
@@ -587,6 +589,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
     return -1;
   }
   auto mp = std::make_unique<MultiPlan>();
+  mp->split = (flags & FLAG_SPLIT_MFMA) != 0;
   struct Tr {
     unsigned idx;
     size_t pts;
@@ -617,7 +620,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
   }
   {
     std::ostringstream sg;
-    sg << "multi3d";
+    sg << (mp->split ? "multi3d_split" : "multi3d");
     for (auto& kv : sizes) sg << ":" << kv.first << "*" << kv.second;
     mp->sig = sg.str();
   }
@@ -687,7 +690,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
   HIP_TRY(hipMemcpy(mp->d_phase, mp->phase_start.data(), sizeof(unsigned) * mp->phase_start.size(),
                     hipMemcpyHostToDevice));
   unsigned per = 0;
-  if (multi_default_grid(&mp->grid, &per)) return -1;
+  if (multi_default_grid(&mp->grid, &per, mp->split)) return -1;
   *handle = mp.release();
   return 0;
 }
@@ -701,14 +704,14 @@ int multi_exec(void* handle, void* stream) {
   if (mp->nphase == 0) return 0;
   if (mp->mode == 0) {
     if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, 0, mp->nphase, mp->d_ctl, mp->grid, mp->chunk,
-                     g_multi_spin_limit, stream)) {
+                     g_multi_spin_limit, stream, mp->split)) {
       set_error("multi-transform launch failed: %s", hipGetErrorString(hipGetLastError()));
       return -1;
     }
   } else {
     for (unsigned ph = 0; ph < mp->nphase; ++ph)
       if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, ph, 1, mp->d_ctl, mp->grid, mp->chunk,
-                       g_multi_spin_limit, stream)) {
+                       g_multi_spin_limit, stream, mp->split)) {
         set_error("multi-transform launch failed: %s", hipGetErrorString(hipGetLastError()));
         return -1;
       }
@@ -749,7 +752,7 @@ int multi_get_schedule(void* handle, int* mode, unsigned* grid, unsigned* chunk,
 int multi_set_schedule(void* handle, int mode, unsigned grid, unsigned chunk) {
   MultiPlan* mp = static_cast<MultiPlan*>(handle);
   unsigned g = 0, per = 0;
-  if (!mp || multi_default_grid(&g, &per)) return -1;
+  if (!mp || multi_default_grid(&g, &per, mp->split)) return -1;
   if ((mode != 0 && mode != 1) || chunk == 0 || grid == 0 || grid > g) {
     set_error("schedule out of range (grid must be 1..%u resident workgroups)", g);
     return -1;
@@ -771,7 +774,7 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
   State& st = S();
   std::lock_guard<std::recursive_mutex> lk(st.mu);
   unsigned gmax = 0, per = 0;
-  if (multi_default_grid(&gmax, &per)) return -1;
+  if (multi_default_grid(&gmax, &per, mp->split)) return -1;
   const unsigned ncu = gmax / per;
   // cached?
   if (cache_path && *cache_path) {

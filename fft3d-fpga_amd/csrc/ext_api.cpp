@@ -59,7 +59,7 @@ int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2* d_in, float2* d_out, bool 
 }
 
 int fftamd_multi_create(fftamd_multi_t* plan, unsigned count, const fftamd_xform3d* xf,
-                        const float2* d_in, float2* d_out, int inverse) {
+                        const float2* d_in, float2* d_out, int inverse, unsigned flags) {
   if (!plan || !xf || count == 0) {
     fftamd::set_error("bad multi-transform arguments");
     return -1;
@@ -76,7 +76,7 @@ int fftamd_multi_create(fftamd_multi_t* plan, unsigned count, const fftamd_xform
     oo[i] = xf[i].out_offset;
   }
   void* h = nullptr;
-  if (fftamd::multi_create(&h, count, dims.data(), io.data(), oo.data(), d_in, d_out, inverse != 0))
+  if (fftamd::multi_create(&h, count, dims.data(), io.data(), oo.data(), d_in, d_out, inverse != 0, flags))
     return -1;
   *plan = reinterpret_cast<fftamd_multi_t>(h);
   return 0;

@@ -31,8 +31,8 @@ struct MultiItem {
 // phases [phase0, phase0 + nphase), `chunk` items per queue ticket).
 int launch_multi(const MultiPass* passes, const MultiItem* items, const unsigned* phase_start,
                  unsigned phase0, unsigned nphase, MultiCtl* ctl, unsigned grid, unsigned chunk,
-                 unsigned spin_limit, void* stream);
+                 unsigned spin_limit, void* stream, bool split);
 // Resident 256-thread workgroups per CU for the persistent kernel.
-int multi_max_blocks_per_cu(int* out);
+int multi_max_blocks_per_cu(int* out, bool split);
 
 }  // namespace fftamd

@@ -19,37 +19,40 @@
 
 namespace fftamd {
 
-template <int MODE, int LG>
+template <int MODE, int LG, bool SPLIT>
 __device__ __forceinline__ void run_path(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
   static_assert(geom_threads(1 << LG, MODE) == MULTI_THREADS, "multi paths use 256-thread tiles");
-  pass_block<float, (1 << LG), MODE, EPI_NONE>(a, blk, lds);
+  // split precision applies to lines with a radix-16 stage
+  pass_block<float, (1 << LG), MODE, EPI_NONE, SPLIT && (LG >= 4)>(a, blk, lds);
 }
 
+template <bool SPLIT>
 __device__ __forceinline__ void dispatch_path(int path, const PassArgs& a, unsigned long long blk,
                                               cpx<float>* lds) {
   switch (path) {
-    case 0: run_path<MODE_CONTIG, 1>(a, blk, lds); break;
-    case 1: run_path<MODE_CONTIG, 2>(a, blk, lds); break;
-    case 2: run_path<MODE_CONTIG, 3>(a, blk, lds); break;
-    case 3: run_path<MODE_CONTIG, 4>(a, blk, lds); break;
-    case 4: run_path<MODE_CONTIG, 5>(a, blk, lds); break;
-    case 5: run_path<MODE_CONTIG, 6>(a, blk, lds); break;
-    case 6: run_path<MODE_CONTIG, 7>(a, blk, lds); break;
-    case 7: run_path<MODE_CONTIG, 8>(a, blk, lds); break;
-    case 8: run_path<MODE_STRIDED, 1>(a, blk, lds); break;
-    case 9: run_path<MODE_STRIDED, 2>(a, blk, lds); break;
-    case 10: run_path<MODE_STRIDED, 3>(a, blk, lds); break;
-    case 11: run_path<MODE_STRIDED, 4>(a, blk, lds); break;
-    case 12: run_path<MODE_STRIDED, 5>(a, blk, lds); break;
-    case 13: run_path<MODE_STRIDED, 6>(a, blk, lds); break;
-    case 14: run_path<MODE_STRIDED, 7>(a, blk, lds); break;
-    case 15: run_path<MODE_STRIDED, 8>(a, blk, lds); break;
+    case 0: run_path<MODE_CONTIG, 1, SPLIT>(a, blk, lds); break;
+    case 1: run_path<MODE_CONTIG, 2, SPLIT>(a, blk, lds); break;
+    case 2: run_path<MODE_CONTIG, 3, SPLIT>(a, blk, lds); break;
+    case 3: run_path<MODE_CONTIG, 4, SPLIT>(a, blk, lds); break;
+    case 4: run_path<MODE_CONTIG, 5, SPLIT>(a, blk, lds); break;
+    case 5: run_path<MODE_CONTIG, 6, SPLIT>(a, blk, lds); break;
+    case 6: run_path<MODE_CONTIG, 7, SPLIT>(a, blk, lds); break;
+    case 7: run_path<MODE_CONTIG, 8, SPLIT>(a, blk, lds); break;
+    case 8: run_path<MODE_STRIDED, 1, SPLIT>(a, blk, lds); break;
+    case 9: run_path<MODE_STRIDED, 2, SPLIT>(a, blk, lds); break;
+    case 10: run_path<MODE_STRIDED, 3, SPLIT>(a, blk, lds); break;
+    case 11: run_path<MODE_STRIDED, 4, SPLIT>(a, blk, lds); break;
+    case 12: run_path<MODE_STRIDED, 5, SPLIT>(a, blk, lds); break;
+    case 13: run_path<MODE_STRIDED, 6, SPLIT>(a, blk, lds); break;
+    case 14: run_path<MODE_STRIDED, 7, SPLIT>(a, blk, lds); break;
+    case 15: run_path<MODE_STRIDED, 8, SPLIT>(a, blk, lds); break;
     default: break;
   }
 }
 
-// largest LDS footprint over the 16 paths (L <= 256, 256 threads)
-constexpr int multi_lds_elems() {
+// largest LDS footprint over the 16 paths (L <= 256, 256 threads), including
+// the per-wave scratch of split-precision stages
+constexpr int multi_lds_elems(bool split) {
   int m = 1;
   for (int lg = 1; lg <= MULTI_MAX_LG; ++lg) {
     const int L = 1 << lg;
@@ -58,7 +61,8 @@ constexpr int multi_lds_elems() {
     m = c0 > m ? c0 : m;
     m = c1 > m ? c1 : m;
   }
-  return m;
+  const int sc = (MULTI_THREADS / 64) * SPLIT_WAVE_FLOATS / 2;
+  return split && sc > m ? sc : m;
 }
 
 __global__ void k_multi_reset(MultiCtl* ctl) {
@@ -68,11 +72,12 @@ __global__ void k_multi_reset(MultiCtl* ctl) {
 }
 
 // 4 waves per SIMD: the 34.8 KB LDS footprint allows 4 workgroups per CU
+template <bool SPLIT>
 __global__ void __launch_bounds__(MULTI_THREADS, 4)
     k_fft_multi(const MultiPass* __restrict__ passes, const MultiItem* __restrict__ items,
                 const unsigned* __restrict__ phase_start, unsigned phase0, unsigned nphase,
                 MultiCtl* ctl, unsigned chunk, unsigned spin_limit) {
-  __shared__ cpx<float> lds[multi_lds_elems()];
+  __shared__ cpx<float> lds[multi_lds_elems(SPLIT)];
   __shared__ unsigned s_ticket;
   __shared__ int s_abort;
   const unsigned nwg = gridDim.x;
@@ -91,7 +96,7 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
         if (t != t0) __syncthreads();
         const MultiItem it = items[base + t];
         const MultiPass* mp = passes + it.pass;
-        dispatch_path(mp->path, mp->a, it.block, lds);
+        dispatch_path<SPLIT>(mp->path, mp->a, it.block, lds);
       }
     }
     if (ph + 1 < nphase) {
@@ -122,18 +127,23 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
 
 int launch_multi(const MultiPass* passes, const MultiItem* items, const unsigned* phase_start,
                  unsigned phase0, unsigned nphase, MultiCtl* ctl, unsigned grid, unsigned chunk,
-                 unsigned spin_limit, void* stream) {
+                 unsigned spin_limit, void* stream, bool split) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_multi_reset, dim3(1), dim3(64), 0, s, ctl);
-  hipLaunchKernelGGL(k_fft_multi, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items, phase_start,
-                     phase0, nphase, ctl, chunk, spin_limit);
+  if (split)
+    hipLaunchKernelGGL(k_fft_multi<true>, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items, phase_start,
+                       phase0, nphase, ctl, chunk, spin_limit);
+  else
+    hipLaunchKernelGGL(k_fft_multi<false>, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items,
+                       phase_start, phase0, nphase, ctl, chunk, spin_limit);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int multi_max_blocks_per_cu(int* out) {
+int multi_max_blocks_per_cu(int* out, bool split) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_multi, MULTI_THREADS, 0) != hipSuccess)
-    return -1;
+  hipError_t e = split ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_multi<true>, MULTI_THREADS, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_multi<false>, MULTI_THREADS, 0);
+  if (e != hipSuccess) return -1;
   *out = n;
   return 0;
 }

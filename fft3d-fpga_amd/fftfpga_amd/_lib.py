@@ -71,7 +71,7 @@ SIGNATURES = {
     "fftamd_z2z_dev": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _u, _vp]),
     "fftamd_c2c_weighted_dev": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _vp, _vp, _vp]),
     "fftfpgaf_c2c_3d_ddr_dev": (_i, [_u, _vp, _vp, _b, _u, _vp]),
-    "fftamd_multi_create": (_i, [ctypes.POINTER(_vp), _u, _vp, _vp, _vp, _i]),
+    "fftamd_multi_create": (_i, [ctypes.POINTER(_vp), _u, _vp, _vp, _vp, _i, _u]),
     "fftamd_multi_exec": (_i, [_vp, _vp]),
     "fftamd_multi_check": (_i, [_vp, _vp]),
     "fftamd_multi_tune": (_i, [_vp, _vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_double)]),

@@ -33,7 +33,8 @@ def packed_offsets(shapes):
 
 
 class MultiPlan:
-    def __init__(self, shapes, d_in, d_out, in_offsets=None, out_offsets=None, inverse=False):
+    def __init__(self, shapes, d_in, d_out, in_offsets=None, out_offsets=None, inverse=False,
+                 split=False):
         if d_in.dtype != torch.complex64 or d_out.dtype != torch.complex64:
             raise TypeError("complex64 buffers expected")
         if not (d_in.is_cuda and d_out.is_cuda and d_in.is_contiguous() and d_out.is_contiguous()):
@@ -55,7 +56,8 @@ class MultiPlan:
         self._h = ctypes.c_void_p()
         rc = self._lib.fftamd_multi_create(ctypes.byref(self._h), len(shapes), ctypes.cast(arr, ctypes.c_void_p),
                                            ctypes.c_void_p(d_in.data_ptr()),
-                                           ctypes.c_void_p(d_out.data_ptr()), 1 if inverse else 0)
+                                           ctypes.c_void_p(d_out.data_ptr()), 1 if inverse else 0,
+                                           0x2 if split else 0)
         if rc != 0:
             raise RuntimeError(f"fftamd_multi_create failed: {last_error()}")
         self._keep = (d_in, d_out)  # the plan holds raw pointers into these tensors

@@ -76,8 +76,9 @@ typedef struct {
   unsigned long long out_offset; /* first element, in complex elements from d_out */
 } fftamd_xform3d;
 
+/* flags: 0 or FFTAMD_SPLIT_MFMA. */
 int fftamd_multi_create(fftamd_multi_t *plan, unsigned count, const fftamd_xform3d *xf,
-                        const float2 *d_in, float2 *d_out, int inverse);
+                        const float2 *d_in, float2 *d_out, int inverse, unsigned flags);
 /* Enqueue the plan on a stream (asynchronous). */
 int fftamd_multi_exec(fftamd_multi_t plan, void *stream);
 /* Synchronise the stream and report whether the last launch's grid barrier

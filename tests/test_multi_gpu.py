@@ -169,3 +169,19 @@ def test_autotuner_picks_and_caches(gpu_lib, tmp_path):
     assert r2["from_cache"] and (r2["mode"], r2["grid"], r2["chunk"]) == (r1["mode"], r1["grid"], r1["chunk"])
     plan2()
     plan2.check()
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_mixed_batch_split_precision(gpu_lib, orc, inverse):
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    host, d_in, offs = _pack(MIXED, 8)
+    d_out = torch.zeros_like(d_in)
+    plan = MultiPlan(MIXED, d_in, d_out, inverse=inverse, split=True)
+    plan()
+    plan.check()
+    out = d_out.cpu().numpy()
+    for s, o in zip(MIXED, offs):
+        n = int(np.prod(s))
+        err = orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s), inverse=inverse))
+        assert err <= TOL, (s, err)
