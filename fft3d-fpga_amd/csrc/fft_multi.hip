@@ -20,10 +20,24 @@
 namespace fftamd {
 
 template <int MODE, int LG, bool SPLIT>
-__device__ __forceinline__ void run_path(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
+__device__ __forceinline__ void run_path_body(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
   static_assert(geom_threads(1 << LG, MODE) == MULTI_THREADS, "multi paths use 256-thread tiles");
   // split precision applies to lines with a radix-16 stage
   pass_block<float, (1 << LG), MODE, EPI_NONE, SPLIT && (LG >= 4)>(a, blk, lds);
+}
+
+// Split-precision paths run as one call per tile: out of line, each gets its
+// own register allocation (inlined together they exceed the 128-VGPR budget
+// of 4 waves per SIMD and spill heavily).  FP32 paths stay inlined.
+template <int MODE, int LG>
+__device__ __noinline__ void run_path_call(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
+  run_path_body<MODE, LG, true>(a, blk, lds);
+}
+
+template <int MODE, int LG, bool SPLIT>
+__device__ __forceinline__ void run_path(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
+  if constexpr (SPLIT) run_path_call<MODE, LG>(a, blk, lds);
+  else run_path_body<MODE, LG, false>(a, blk, lds);
 }
 
 template <bool SPLIT>

@@ -34,14 +34,15 @@ typedef float split_float4 __attribute__((ext_vector_type(4)));
 constexpr int SPLIT_ROW = 36;                      // floats per scratch row
 constexpr int SPLIT_WAVE_FLOATS = 64 * SPLIT_ROW;  // per wave
 
-// cos(2 pi m / 16) for m mod 16, from the exact constants of the radix-16 butterfly
+// cos(2 pi m / 16) for m mod 16, from the exact constants of the radix-16
+// butterfly (selects only: no indexed local array, which would live in scratch)
 __device__ __forceinline__ float cos16(int m) {
   m &= 15;
-  const float c1 = K<float>::C16, c2 = K<float>::C8, c3 = K<float>::S16;
-  const float t[5] = {1.0f, c1, c2, c3, 0.0f};
   // cos is even and cos(pi - x) = -cos(x)
-  int a = m <= 8 ? m : 16 - m;  // 0..8
-  return a <= 4 ? t[a] : -t[8 - a];
+  const int a = m <= 8 ? m : 16 - m;  // 0..8
+  const int b = a <= 4 ? a : 8 - a;   // 0..4
+  const float v = b == 0 ? 1.0f : b == 1 ? K<float>::C16 : b == 2 ? K<float>::C8 : b == 3 ? K<float>::S16 : 0.0f;
+  return a <= 4 ? v : -v;
 }
 
 __device__ __forceinline__ void split2(float x, _Float16* hi, _Float16* lo) {
@@ -83,17 +84,19 @@ __device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  split_float4 yr[4], yi[4];
-  float scl[4];
+  // Group g reads rows 16g..16g+15 and writes its results back into the same
+  // rows: the wave issues every read of the group before any of its writes,
+  // and one wave's LDS operations complete in order, so each group is done in
+  // place and only one group's values are live at a time.
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     // operand slice: vector (16g + row), values 8*part .. 8*part+7 of [re 0..15 | im 0..15]
-    const float* src = scratch + (16 * g + row) * SPLIT_ROW + 8 * part;
+    float* rowp = scratch + (16 * g + row) * SPLIT_ROW;
     float x[8];
     float mx = 0.0f;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      x[j] = src[j];
+      x[j] = rowp[8 * part + j];
       mx = fmaxf(mx, fabsf(x[j]));
     }
 #pragma unroll
@@ -103,7 +106,7 @@ __device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) 
     (void)frexpf(mx, &ex);  // mx = f * 2^ex, f in [0.5, 1)
     const int sh = mx > 0.0f ? 15 - ex : 0;
     const float s = ldexpf(1.0f, sh);
-    scl[g] = ldexpf(1.0f, -sh);
+    const float inv_s = ldexpf(1.0f, -sh);
     split_half8 bh, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -112,27 +115,21 @@ __device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) 
       bh[j] = h;
       bl[j] = l;
     }
-    split_float4 z = {0.f, 0.f, 0.f, 0.f};
+    const split_float4 z = {0.f, 0.f, 0.f, 0.f};
     split_float4 r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bh, z, 0, 0, 0);
-    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bl, r, 0, 0, 0);
-    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, bh, r, 0, 0, 0);
     split_float4 q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh, z, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bl, r, 0, 0, 0);
     q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl, q, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, bh, r, 0, 0, 0);
     q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh, q, 0, 0, 0);
-    yr[g] = r;
-    yi[g] = q;
-  }
-  // ---- results back to one vector per lane: lane holds points 4*part+e of vector 16g+row
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    float* dst = scratch + (16 * g + row) * SPLIT_ROW;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // this lane now holds points 4*part+e of vector 16g+row
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      dst[4 * part + e] = yr[g][e] * scl[g];
-      dst[16 + 4 * part + e] = yi[g][e] * scl[g];
+      rowp[4 * part + e] = r[e] * inv_s;
+      rowp[16 + 4 * part + e] = q[e] * inv_s;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
