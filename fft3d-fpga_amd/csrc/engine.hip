@@ -152,10 +152,10 @@ static std::vector<R> stockham_table(int L) {
   for (int st = 1; st < np; ++st) {
     const int rad = sched_radix(L, st), ns = sched_ns(L, st), off = sched_twoff(L, st);
     for (int k = 0; k < ns; ++k)
-      for (int r = 1; r < rad; ++r) {
+      for (int q = 0; q < TW_PER_K; ++q) {  // W_{16 NS}^{k 2^q}
         long double c, s;
-        twiddle_ld((long long)k * r, (long long)ns * rad, &c, &s);
-        size_t i = (size_t)off + (size_t)k * (rad - 1) + (r - 1);
+        twiddle_ld((long long)k << q, (long long)ns * rad, &c, &s);
+        size_t i = (size_t)off + (size_t)TW_PER_K * k + q;
         v[2 * i] = (R)c;
         v[2 * i + 1] = (R)s;
       }

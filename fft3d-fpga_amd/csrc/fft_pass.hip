@@ -21,7 +21,7 @@ template <typename R, int L, int MODE, int EPI, bool SPLIT>
 __global__ void __launch_bounds__(geom_threads(L, MODE))
     k_fft_pass(const PassArgs a) {
   using G = Geom<L, MODE>;
-  constexpr bool NEED_LDS = G::NP > 1 || EPI == EPI_BITREV || SPLIT;
+  constexpr bool NEED_LDS = G::NP > 1 || SPLIT;
   __shared__ cpx<R> lds[NEED_LDS ? (SPLIT ? G::LDS_ELEMS_SPLIT : G::LDS_ELEMS) : 1];
   pass_block<R, L, MODE, EPI, SPLIT>(a, blockIdx.x, lds);
 }

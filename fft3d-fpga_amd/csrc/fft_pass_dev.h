@@ -48,14 +48,34 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* 
   constexpr int NS = sched_ns(L, S);
   constexpr int NB = G::PPT / RAD;
   if constexpr (S > 0) {
+    static_assert(RAD == 16, "stages after the first are radix 16");
     constexpr int OFF = sched_twoff(L, S);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int j = tau + b * G::TPL;
       const int k = j & (NS - 1);
-      const cpx<R>* t = tw + OFF + k * (RAD - 1);
+      // W^k, W^2k, W^4k, W^8k from the table; W^{kr} for the other r by products
+      const cpx<R>* t = tw + OFF + TW_PER_K * k;
+      const cpx<R> w1 = t[0], w2 = t[1], w4 = t[2], w8 = t[3];
+      const cpx<R> w3 = cmul(w1, w2), w12 = cmul(w4, w8);
+      cpx<R> w[16];
+      w[1] = w1;
+      w[2] = w2;
+      w[3] = w3;
+      w[4] = w4;
+      w[5] = cmul(w4, w1);
+      w[6] = cmul(w4, w2);
+      w[7] = cmul(w4, w3);
+      w[8] = w8;
+      w[9] = cmul(w8, w1);
+      w[10] = cmul(w8, w2);
+      w[11] = cmul(w8, w3);
+      w[12] = w12;
+      w[13] = cmul(w12, w1);
+      w[14] = cmul(w12, w2);
+      w[15] = cmul(w12, w3);
 #pragma unroll
-      for (int r = 1; r < RAD; ++r) v[b * RAD + r] = cmul(v[b * RAD + r], t[r - 1]);
+      for (int r = 1; r < 16; ++r) v[b * RAD + r] = cmul(v[b * RAD + r], w[r]);
     }
   }
   if constexpr (SPLIT && RAD == 16 && sizeof(R) == 4) {
@@ -174,23 +194,22 @@ __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long
   constexpr int NSL = sched_ns(L, NP - 1);
   constexpr int NBL = PPT / RL;
   if constexpr (EPI == EPI_BITREV) {
-    // stage through LDS so the bit-reversed line is written with coalesced stores
-    if constexpr (NP > 1 || SPLIT) __syncthreads();
+    // Point k = j + r*NSL (r < RL) lands at bitrev(k) = bitrev(r) + RL*bitrev(j):
+    // each butterfly's RL outputs form one contiguous run of the output line,
+    // so the permuted line is written straight from registers in RL*8-byte runs.
+    constexpr unsigned LGR = ilog2c(RL), LGJ = ilog2c(L) - ilog2c(RL);
 #pragma unroll
     for (int b = 0; b < NBL; ++b) {
       const int j = tau + b * TPL;
+      const long long run = (long long)RL * brev_bits((unsigned)j, LGJ);
 #pragma unroll
-      for (int r = 0; r < RL; ++r) lds[G::idx(c, j + r * NSL)] = v[b * RL + r];
-    }
-    __syncthreads();
-    constexpr unsigned LG = ilog2c(L);
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const int q = tau + m * TPL;
-      cpx<R> x = lds[G::idx(c, (int)brev_bits((unsigned)q, LG))];
-      if (a.inverse) x = cswap(x);
-      if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + (long long)q * a.out_p) & a.w_mask]);
-      if (active) out[(long long)q * a.out_p] = x;
+      for (int r = 0; r < RL; ++r) {
+        const long long q = run + brev_bits((unsigned)r, LGR);
+        cpx<R> x = v[b * RL + r];
+        if (a.inverse) x = cswap(x);
+        if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + q * a.out_p) & a.w_mask]);
+        if (active) out[q * a.out_p] = x;
+      }
     }
   } else {
 #pragma unroll

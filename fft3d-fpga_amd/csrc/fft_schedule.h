@@ -29,11 +29,13 @@ constexpr int sched_ns(int L, int s) {
   for (int i = 0; i < s; ++i) n *= sched_radix(L, i);
   return n;
 }
-// Twiddles of stage s >= 1 sit at [twoff(s) + k*(R-1) + (r-1)], k < NS(s), 1 <= r < R,
-// holding W_{NS*R}^{k*r}.
+// Stages s >= 1 are radix 16.  Their twiddles sit at [twoff(s) + 4k + q],
+// k < NS(s), holding W_{16 NS}^{k * 2^q} for q = 0..3; the other powers
+// W^{k r} (r = 1..15) are formed by at most three complex products.
+constexpr int TW_PER_K = 4;
 constexpr int sched_twoff(int L, int s) {
   int o = 0;
-  for (int i = 1; i < s; ++i) o += sched_ns(L, i) * (sched_radix(L, i) - 1);
+  for (int i = 1; i < s; ++i) o += sched_ns(L, i) * TW_PER_K;
   return o;
 }
 constexpr int sched_twsize(int L) { return sched_twoff(L, sched_np(L)); }
@@ -43,7 +45,8 @@ constexpr int geom_ppt(int L) { return L < 16 ? L : 16; }
 constexpr int geom_tpl(int L) { return L / geom_ppt(L); }
 // Lines per workgroup.  Contiguous lines: 256 threads per workgroup (one line
 // per workgroup from L = 4096 up).  Strided lines: at least 16 adjacent
-// columns (128-byte row segments) and 256 threads up to L = 256.
+// columns (128-byte row segments; narrower segments measured far slower) and
+// 256 threads up to L = 256.
 constexpr int geom_cols(int L, int mode) {
   return mode == MODE_CONTIG ? (geom_tpl(L) >= 256 ? 1 : 256 / geom_tpl(L))
                              : (L >= 256 ? 16 : 256 / geom_tpl(L));

@@ -198,3 +198,19 @@ def test_plan_shapes_of_headline_configs(fl):
     for dims in ([2, 2, 2], [8192], [512, 512], [1 << 22]):
         for q in describe(fl, dims, 1)[0]:
             assert q["NT"] <= 1024
+
+
+@pytest.mark.parametrize("lg", range(1, 14))
+def test_bitrev_epilogue_runs(lg):
+    """The bit-reversed epilogue writes point k = j + r*NS of the last stage to
+    RL*bitrev(j) + bitrev(r); that must equal bitrev(k) over log2 L bits."""
+    L = 1 << lg
+    _, rad, ns = sched(L)
+    RL, NSL = rad[-1], ns[-1]
+    lgr = RL.bit_length() - 1
+    j = np.arange(NSL)[:, None]
+    r = np.arange(RL)[None, :]
+    k = j + r * NSL
+    got = RL * brev(j, lg - lgr) + brev(r, lgr)
+    assert np.array_equal(got, brev(k, lg))
+    assert np.unique(got).size == L
