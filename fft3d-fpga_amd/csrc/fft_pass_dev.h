@@ -139,35 +139,68 @@ __device__ __forceinline__ void run_stages(cpx<R>* v, cpx<R>* lds, int c, int ta
   }
 }
 
-// One workgroup tile of a pass: block index g (decoded into column block,
-// middle and outer line index).  Used by k_fft_pass (one tile per workgroup)
-// and by the persistent multi-transform kernel (many tiles per workgroup).
-template <typename R, int L, int MODE, int EPI, bool SPLIT = false>
-__device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds) {
+// Where this thread's line of tile g sits (tile = column block, middle and
+// outer line index decoded from g).
+struct TileLoc {
+  int c, tau;
+  unsigned col;
+  bool active;
+  long long in_off, out_off;
+};
+
+template <int L, int MODE>
+__device__ __forceinline__ TileLoc tile_locate(const PassArgs& a, unsigned long long g) {
   using G = Geom<L, MODE>;
-  constexpr int PPT = G::PPT, TPL = G::TPL, C = G::C;
-  constexpr int NP = G::NP;
-
+  TileLoc t;
   const int tid = threadIdx.x;
-  const int c = MODE == MODE_CONTIG ? tid / TPL : tid % C;
-  const int tau = MODE == MODE_CONTIG ? tid % TPL : tid / C;
-
+  t.c = MODE == MODE_CONTIG ? tid / G::TPL : tid % G::C;
+  t.tau = MODE == MODE_CONTIG ? tid % G::TPL : tid / G::C;
   const unsigned cb = (unsigned)(g % a.ncb);
   g /= a.ncb;
   const unsigned kk = (unsigned)(g % a.K);
   const unsigned long long o = g / a.K + a.outer0;
-  const unsigned col = cb * C + c;
-  const bool active = col < a.ncol;
+  t.col = cb * G::C + t.c;
+  t.active = t.col < a.ncol;
   const unsigned ko = a.kbits ? brev_bits(kk, a.kbits) : kk;
+  t.in_off = (long long)o * a.in_o + (long long)kk * a.in_k + (long long)t.col * a.in_c;
+  t.out_off = (long long)o * a.out_o + (long long)ko * a.out_k + (long long)t.col * a.out_c;
+  return t;
+}
 
-  const long long in_off = (long long)o * a.in_o + (long long)kk * a.in_k + (long long)col * a.in_c;
-  const long long out_off = (long long)o * a.out_o + (long long)ko * a.out_k + (long long)col * a.out_c;
-  const cpx<R>* __restrict__ in = reinterpret_cast<const cpx<R>*>(a.in) + in_off;
+// Raw stage-0 inputs of this thread straight from HBM (no arithmetic, so a
+// pipelined kernel can issue them one tile ahead).
+template <typename R, int L, int MODE>
+__device__ __forceinline__ void tile_load(const PassArgs& a, const TileLoc& t, cpx<R>* v) {
+  using G = Geom<L, MODE>;
+  constexpr int RAD = sched_radix(L, 0);
+  constexpr int NB = G::PPT / RAD;
+  const cpx<R>* __restrict__ in = reinterpret_cast<const cpx<R>*>(a.in) + t.in_off;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t.tau + b * G::TPL;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) {
+      const long long p = j + r * (L / RAD);
+      cpx<R> x = {R(0), R(0)};
+      if (t.active) x = in[p * a.in_p];
+      v[b * RAD + r] = x;
+    }
+  }
+}
+
+// Input weights and direction, radix stages and the epilogue of one tile
+// whose raw inputs are in v.
+template <typename R, int L, int MODE, int EPI, bool SPLIT = false>
+__device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t, cpx<R>* v, cpx<R>* lds) {
+  using G = Geom<L, MODE>;
+  constexpr int PPT = G::PPT, TPL = G::TPL;
+  constexpr int NP = G::NP;
+  const int c = t.c, tau = t.tau;
+  const unsigned col = t.col;
+  const bool active = t.active;
+  const long long in_off = t.in_off, out_off = t.out_off;
   cpx<R>* __restrict__ out = reinterpret_cast<cpx<R>*>(a.out) + out_off;
   const cpx<R>* __restrict__ tw = reinterpret_cast<const cpx<R>*>(a.tw);
-
-  cpx<R> v[PPT];
-  // ---- stage 0 inputs straight from HBM
   {
     constexpr int RAD = sched_radix(L, 0);
     constexpr int NB = PPT / RAD;
@@ -176,11 +209,10 @@ __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long
       const int j = tau + b * TPL;
 #pragma unroll
       for (int r = 0; r < RAD; ++r) {
-        const long long p = j + r * (L / RAD);
-        cpx<R> x = {R(0), R(0)};
-        if (active) {
-          x = in[p * a.in_p];
-          if (a.w_load) x = cscale(x, (R)a.w_load[(unsigned long long)(in_off + p * a.in_p) & a.w_mask]);
+        cpx<R> x = v[b * RAD + r];
+        if (a.w_load && active) {
+          const long long p = j + r * (L / RAD);
+          x = cscale(x, (R)a.w_load[(unsigned long long)(in_off + p * a.in_p) & a.w_mask]);
         }
         v[b * RAD + r] = a.inverse ? cswap(x) : x;
       }
@@ -234,6 +266,16 @@ __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long
       }
     }
   }
+}
+
+// One workgroup tile of a pass (block index g).  Used by k_fft_pass (one tile
+// per workgroup) and by the persistent multi-transform kernel.
+template <typename R, int L, int MODE, int EPI, bool SPLIT = false>
+__device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds) {
+  cpx<R> v[Geom<L, MODE>::PPT];
+  const TileLoc t = tile_locate<L, MODE>(a, g);
+  tile_load<R, L, MODE>(a, t, v);
+  tile_finish<R, L, MODE, EPI, SPLIT>(a, t, v, lds);
 }
 
 }  // namespace fftamd

@@ -24,7 +24,8 @@ HEADERS = [os.path.join(INC, "fftfpga", h) for h in ("fftfpga.h", "fftfpga_hip.h
 
 def declared_functions():
     names = set()
-    pat = re.compile(r"^\s*(?:extern\s+)?(?:const\s+)?[A-Za-z_]\w*\s*\*?\s*\*?([A-Za-z_]\w*)\s*\(")
+    # return type of one or more words and stars, then the name and "("
+    pat = re.compile(r"^\s*(?:extern\s+)?(?:[A-Za-z_]\w*[\s*]+)+([A-Za-z_]\w*)\s*\(")
     for h in HEADERS:
         for line in open(h):
             if line.lstrip().startswith(("*", "/*", "//", "#")):
