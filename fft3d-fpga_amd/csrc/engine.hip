@@ -27,6 +27,7 @@
 #include "engine.h"
 #include "fft_multi.h"
 #include "fft_pass.h"
+#include "fft_xy.h"
 #include "fft_schedule.h"
 
 namespace fftamd {
@@ -98,6 +99,7 @@ struct State {
   size_t h_stage_cap = 0;
   std::map<PlanKey, Plan> plans;
   std::map<std::pair<int, int>, void*> tw_stockham;  // (prec, L)
+  void* tw_full256 = nullptr;                         // W_256^j, FP32
   std::map<std::pair<int, long long>, std::pair<void*, void*>> tw_fourstep;  // (prec, M)
 };
 
@@ -183,6 +185,26 @@ static int get_stockham(int prec, int L, const void** out) {
   }
   st.tw_stockham[key] = d;
   *out = d;
+  return 0;
+}
+
+// W_256^j for j = 0..255 (FP32), for the fused plane pass
+static int get_full256(const void** out) {
+  State& st = S();
+  if (!st.tw_full256) {
+    std::vector<float> h(2 * 256);
+    for (int j = 0; j < 256; ++j) {
+      long double c, s;
+      twiddle_ld(j, 256, &c, &s);
+      h[2 * j] = (float)c;
+      h[2 * j + 1] = (float)s;
+    }
+    void* d = nullptr;
+    HIP_TRY(hipMalloc(&d, h.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    st.tw_full256 = d;
+  }
+  *out = st.tw_full256;
   return 0;
 }
 
@@ -276,7 +298,26 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   const bool bitrev = (flags & FLAG_BITREV_1D) && rank == 1;
   std::vector<PassDesc> ps;
 
-  for (int a = rank - 1; a >= 0; --a) {
+  // 256 x 256 planes (the two fastest axes): one fused x+y pass
+  int top = rank - 1;
+  if (rank >= 2 && dims[rank - 1] == 256 && dims[rank - 2] == 256 && prec == PREC_F32 && !plan.split &&
+      !(flags & FLAG_NO_PLANE_FUSION)) {
+    unsigned long long O = batch;
+    for (int d = 0; d < rank - 2; ++d) O *= dims[d];
+    PassDesc p;
+    p.lg = 8;
+    p.mode = MODE_PLANE256;
+    p.transposing = true;  // out of place: the four workgroups of a plane all read all of it
+    p.in_p = p.out_p = 1;
+    p.in_c = p.out_c = 256;
+    p.ncol = 256;
+    p.nouter = O;
+    p.in_o = p.out_o = 256 * 256;
+    ps.push_back(p);
+    top = rank - 3;
+  }
+
+  for (int a = top; a >= 0; --a) {
     const long long L = (long long)dims[a];
     if (L == 1) continue;
     long long s = 1;
@@ -408,6 +449,7 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
       int dst;
       if (i == 0) dst = first_dst;
       else if (ps[i].transposing) dst = cur == B_OUT ? B_TMP : B_OUT;
+      else if (i + 1 == ps.size()) dst = B_OUT;  // a line pass may also run out of place
       else dst = cur;
       ps[i].src = cur;
       ps[i].dst = dst;
@@ -439,7 +481,7 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
 static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
                     Plan** out) {
   State& st = S();
-  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & (FLAG_BITREV_1D | FLAG_SPLIT_MFMA),
+  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & (FLAG_BITREV_1D | FLAG_SPLIT_MFMA | FLAG_NO_PLANE_FUSION),
               prec, aliased);
   auto it = st.plans.find(key);
   if (it == st.plans.end()) {
@@ -468,6 +510,33 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
   int cur = B_IN;
   for (size_t pi = 0; pi < plan.passes.size(); ++pi) {
     const PassDesc& p = plan.passes[pi];
+    if (p.mode == MODE_PLANE256) {
+      XYArgs xa{};
+      xa.in = ptr(p.src);
+      xa.out = ptr(p.dst);
+      xa.nplanes = p.nouter;
+      xa.plane_stride = p.in_o;
+      if (get_stockham(plan.prec, 256, &xa.tw) || get_full256(&xa.tw_full)) return -1;
+      xa.w_load = pi == 0 ? w_load : nullptr;
+      xa.w_store = pi + 1 == plan.passes.size() ? w_store : nullptr;
+      xa.w_mask = (unsigned long long)plan.per - 1ULL;
+      xa.inverse = inverse ? 1 : 0;
+      Prof& pf = P();
+      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
+      const int rc = launch_fft_xy256(xa, stream);
+      if (prof) {
+        HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
+        pf.recs.push_back({"fft_plane_f32_256x256", pf.pool[pf.used], pf.pool[pf.used + 1]});
+        pf.used += 2;
+      }
+      if (rc) {
+        set_error("plane pass launch failed (rc=%d): %s", rc, hipGetErrorString(hipGetLastError()));
+        return -1;
+      }
+      cur = p.dst;
+      continue;
+    }
     PassArgs a{};
     a.w_load = pi == 0 ? w_load : nullptr;
     a.w_store = pi + 1 == plan.passes.size() ? w_store : nullptr;
@@ -915,6 +984,8 @@ static void release_locked() {
     (void)hipFree(kv.second.second);
   }
   st.tw_fourstep.clear();
+  if (st.tw_full256) (void)hipFree(st.tw_full256);
+  st.tw_full256 = nullptr;
   if (st.d_in) (void)hipFree(st.d_in);
   if (st.d_out) (void)hipFree(st.d_out);
   if (st.h_stage) (void)hipHostFree(st.h_stage);
@@ -1052,7 +1123,8 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
     long long v[NF] = {d.lg, d.mode, d.epi, d.transposing, d.in_p, d.out_p, d.in_c, d.out_c,
                        d.in_k, d.out_k, d.in_o, d.out_o, d.ncol, d.K, d.kbits,
                        (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
-                       geom_cols(1 << d.lg, d.mode), geom_threads(1 << d.lg, d.mode)};
+                       d.mode == MODE_PLANE256 ? 64 : geom_cols(1 << d.lg, d.mode),
+                       d.mode == MODE_PLANE256 ? 1024 : geom_threads(1 << d.lg, d.mode)};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }
   out[NF * n] = p.final_copy ? 1 : 0;
@@ -1277,7 +1349,9 @@ int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din
     return -1;
   }
   Plan* plan = nullptr;
-  if (get_plan(rank, dims, batch, 0u, PREC_F32, din == dout, &plan)) return -1;
+  // the fused plane pass reads its input in 16-byte pieces
+  const unsigned fl = ((uintptr_t)din & 15u) ? FLAG_NO_PLANE_FUSION : 0u;
+  if (get_plan(rank, dims, batch, fl, PREC_F32, din == dout, &plan)) return -1;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
   return run_plan(*plan, din, dout, inverse, s, w_load, w_store);
 }
@@ -1294,6 +1368,7 @@ int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* d
   }
   Plan* plan = nullptr;
   const bool aliased = din == dout;
+  if ((uintptr_t)din & 15u) flags |= FLAG_NO_PLANE_FUSION;  // the plane pass reads 16-byte pieces
   if (get_plan(rank, dims, batch, flags, prec, aliased, &plan)) return -1;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
   return run_plan(*plan, din, dout, inverse, s);

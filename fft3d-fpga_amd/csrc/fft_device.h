@@ -89,8 +89,9 @@ __device__ __forceinline__ void dft8(cpx<R>* v) {
 // 16-point DFT as 4 x 4: n = 4*n2 + n1, k = k1 + 4*k2.
 //   Y[n1][k1] = DFT4_{n2}(x[4 n2 + n1]);  Y'[n1][k1] = Y[n1][k1] W16^{n1 k1}
 //   X[k1 + 4 k2] = DFT4_{n1}(Y'[n1][k1])
+// First half of dft16: leaves Y'[n1][k1] in v[n1 + 4 k1].
 template <typename R>
-__device__ __forceinline__ void dft16(cpx<R>* v) {
+__device__ __forceinline__ void dft16_head(cpx<R>* v) {
 #pragma unroll
   for (int n1 = 0; n1 < 4; ++n1) dft4s<R, 4>(v + n1);  // v[n1 + 4 k1] = Y[n1][k1]
   const cpx<R> w1 = {K<R>::C16, -K<R>::S16};
@@ -108,6 +109,11 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   v[7] = cmul(v[7], w3);
   v[11] = mul_w8_3(v[11]);
   v[15] = cmul(v[15], w9);
+}
+
+template <typename R>
+__device__ __forceinline__ void dft16(cpx<R>* v) {
+  dft16_head(v);
   cpx<R> t[16];
 #pragma unroll
   for (int k1 = 0; k1 < 4; ++k1) {

@@ -11,7 +11,8 @@
 
 namespace fftamd {
 
-enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1 };
+// MODE_PLANE256: fused x+y pass over 256 x 256 planes (fft_xy.hip)
+enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1, MODE_PLANE256 = 2 };
 // EPI_TWIDDLE multiplies point k of column c by W_M^{(c >> cshift) * k}
 // (four-step split of a long axis); EPI_BITREV stores point k at bitrev(k).
 enum PassEpilogue { EPI_NONE = 0, EPI_TWIDDLE = 1, EPI_BITREV = 2 };

@@ -20,6 +20,7 @@ from ._lib import FpgaT, last_error, load
 
 FFTAMD_BITREV_1D = 0x1
 FFTAMD_SPLIT_MFMA = 0x2
+FFTAMD_NO_PLANE_FUSION = 0x4
 
 
 def _ptr(a, dtype, need: int):
@@ -139,10 +140,11 @@ def _stream_handle(t) -> ctypes.c_void_p:
 
 
 def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=None,
-            split: bool = False):
+            split: bool = False, plane_fusion: bool = True):
     """Transform the last `rank` dims of a contiguous complex64/complex128 GPU
     tensor; leading dims are the batch.  Returns `out` (allocated if None).
-    split=True selects the split-precision FP16 matrix-core radix-16 stages."""
+    split=True selects the split-precision FP16 matrix-core radix-16 stages;
+    plane_fusion=False runs 256 x 256 planes as separate x and y passes."""
     import torch
     if not x.is_cuda:
         raise ValueError("c2c_dev needs a GPU tensor")
@@ -164,7 +166,8 @@ def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=No
         return out
     cd = (ctypes.c_uint * rank)(*dims)
     fn = load().fftamd_c2c_dev if x.dtype == torch.complex64 else load().fftamd_z2z_dev
-    flags = (FFTAMD_BITREV_1D if bitrev_1d else 0) | (FFTAMD_SPLIT_MFMA if split else 0)
+    flags = ((FFTAMD_BITREV_1D if bitrev_1d else 0) | (FFTAMD_SPLIT_MFMA if split else 0)
+             | (0 if plane_fusion else FFTAMD_NO_PLANE_FUSION))
     rc = fn(rank, cd, batch, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
             1 if inverse else 0, flags, _stream_handle(x))
     if rc != 0:

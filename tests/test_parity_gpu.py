@@ -299,3 +299,71 @@ def test_device_api_orders_with_torch_default_stream(gpu_lib):
         d = y - torch.fft.fftn(x, dim=(-3, -2, -1))
         rel = (torch.linalg.vector_norm(d) / torch.linalg.vector_norm(y)).item()
         assert rel < 5e-6, (B, rel)
+
+
+# ---- fused 256 x 256 plane pass ------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape,rank", [((1, 256, 256), 2), ((9, 256, 256), 2), ((4, 256, 256), 3),
+                                        ((2, 8, 256, 256), 3)])
+def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
+    """Plane counts below, at and above one per XCD slot (1, 4, 9, 16); out of
+    place and in place; against the FP64 oracle and the unfused three-pass path."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    n = int(np.prod(shape))
+    x = orc.uniform(n, 900 + len(shape) + shape[0])
+    dims = list(shape[len(shape) - rank:])
+    batch = n // int(np.prod(dims))
+    xt = torch.from_numpy(x.reshape(shape)).cuda()
+    for inv in (False, True):
+        ref = orc.fftnd(x, dims, batch=batch, inverse=inv)
+        y = c2c_dev(xt, rank, inverse=inv)
+        y3 = c2c_dev(xt, rank, inverse=inv, plane_fusion=False)
+        z = xt.clone()
+        c2c_dev(z, rank, inverse=inv, out=z)
+        torch.cuda.synchronize()
+        err = orc.rel_l2(y.cpu().numpy(), ref)
+        err3 = orc.rel_l2(y3.cpu().numpy(), ref)
+        errz = orc.rel_l2(z.cpu().numpy(), ref)
+        print(f"plane {shape} rank={rank} inv={inv} fused={err:.3e} unfused={err3:.3e} inplace={errz:.3e}")
+        assert err <= TOL and err3 <= TOL and errz <= TOL
+    assert np.array_equal(xt.cpu().numpy().reshape(-1), x)  # input untouched
+
+
+@pytest.mark.parametrize("shape,rank", [((2, 256, 256), 2), ((4, 256, 256), 3)])
+def test_plane_pass_weights(gpu_lib, orc, shape, rank):
+    """Weights applied in the fused pass's loads (both ranks) and stores (rank 2)."""
+    import torch
+    from fftfpga_amd.autograd import weighted_transform
+    n = int(np.prod(shape))
+    dims = list(shape[len(shape) - rank:])
+    per = int(np.prod(dims))
+    x = orc.uniform(n, 31)
+    rng = np.random.default_rng(7)
+    w_in = rng.uniform(0.5, 2.0, per).astype(np.float32)
+    w_out = rng.uniform(0.5, 2.0, per).astype(np.float32)
+    xt = torch.from_numpy(x.reshape(shape)).cuda()
+    for inv in (False, True):
+        y = weighted_transform(xt, rank, inverse=inv, w_in=torch.from_numpy(w_in.reshape(dims)).cuda(),
+                               w_out=torch.from_numpy(w_out.reshape(dims)).cuda())
+        torch.cuda.synchronize()
+        xw = (x.reshape(-1, per) * w_in).reshape(-1)
+        ref = (orc.fftnd(xw, dims, batch=n // per, inverse=inv).reshape(-1, per) * w_out).reshape(-1)
+        err = orc.rel_l2(y.cpu().numpy().reshape(-1), ref)
+        print(f"plane weights {shape} rank={rank} inv={inv} rel_l2={err:.3e}")
+        assert err <= TOL
+
+
+def test_plane_pass_unaligned_input(gpu_lib, orc):
+    """An input that is only 8-byte aligned (a view at an odd element offset)
+    takes the separate x and y passes."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    x = orc.uniform(2 * 65536, 44)
+    base = torch.zeros(2 * 65536 + 1, dtype=torch.complex64, device="cuda")
+    base[1:] = torch.from_numpy(x).cuda()
+    xt = base[1:].view(2, 256, 256)
+    assert xt.data_ptr() % 16 == 8
+    y = c2c_dev(xt, 2)
+    torch.cuda.synchronize()
+    assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [256, 256], batch=2)) <= TOL

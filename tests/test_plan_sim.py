@@ -10,6 +10,9 @@
    element exactly once, and the composition must equal the oracle's
    transform (four-step twiddles, bit-reversed 1D order and buffer roles
    included).
+3. The fused 256 x 256 plane pass (fft_xy.hip): its lane, shuffle and LDS
+   index maps and its workgroup-to-plane mapping are replayed and must give
+   the 2D DFT of every plane with every output written once.
 """
 import ctypes
 
@@ -112,6 +115,14 @@ def run_plan(passes, final_copy, x, inverse, aliased=False):
     for p in passes:
         L = 1 << p["lg"]
         C = p["C"]
+        if p["mode"] == 2:  # fused 256 x 256 plane pass, out of place
+            assert p["src"] != p["dst"] and p["transposing"]
+            assert p["nouter"] * p["in_o"] == total and p["in_o"] == 65536
+            planes = bufs[p["src"]].reshape(p["nouter"], 256, 256)
+            X = np.fft.ifft2(planes) * 65536 if inverse else np.fft.fft2(planes)
+            bufs[p["dst"]][:] = X.reshape(-1)
+            cur = p["dst"]
+            continue
         ncb = (p["ncol"] + C - 1) // C
         # every active line (o, k, col); the grid covers col < ncb*C, inactive ones store nothing
         o, k, col = np.meshgrid(np.arange(p["nouter"]), np.arange(p["K"]), np.arange(ncb * C),
@@ -155,7 +166,8 @@ def run_plan(passes, final_copy, x, inverse, aliased=False):
 CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16], 3, 0),
          ([1024, 8], 1, 0), ([2048, 32], 1, 0), ([8, 4096], 1, 0), ([1, 1, 1], 2, 0),
          ([2, 1, 4], 1, 0), ([256], 5, 1), ([1 << 14], 2, 1), ([1 << 14], 2, 0), ([1 << 16], 1, 1),
-         ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0)]
+         ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0), ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
+         ([256, 256], 1, 4), ([2, 256, 256], 1, 4)]
 
 
 @pytest.mark.parametrize("dims,batch,flags", CASES)
@@ -165,7 +177,7 @@ def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
     n = int(np.prod(dims)) * batch
     x = orc.uniform(n, 77 + n % 101)
     got = run_plan(passes, final_copy, x, inverse)
-    if flags & 1:
+    if flags & 1 and len(dims) == 1:
         ref = orc.c2c_1d_expected(x, dims[0], batch, inverse=inverse).reshape(-1)
     else:
         ref = orc.fftnd(x, dims, batch=batch, inverse=inverse).reshape(-1)
@@ -173,7 +185,8 @@ def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
 
 
 @pytest.mark.parametrize("dims,batch,flags", [([64, 64, 64], 1, 0), ([2048, 32], 1, 0),
-                                              ([1 << 14], 1, 1), ([1 << 14], 1, 0)])
+                                              ([1 << 14], 1, 1), ([1 << 14], 1, 0),
+                                              ([2, 256, 256], 1, 0), ([256, 256], 2, 0)])
 def test_plan_replay_in_place(fl, orc, dims, batch, flags):
     passes, final_copy = describe(fl, dims, batch, flags, aliased=True)
     n = int(np.prod(dims)) * batch
@@ -185,9 +198,18 @@ def test_plan_replay_in_place(fl, orc, dims, batch, flags):
 
 
 def test_plan_shapes_of_headline_configs(fl):
-    # 256^3: three single passes (x contiguous, y and z strided), no temporary buffer
+    # 256^3: the fused x+y plane pass, then the strided z pass, no temporary buffer
     p, fc = describe(fl, [256, 256, 256], 1)
+    assert [q["mode"] for q in p] == [2, 1] and not fc and all(q["dst"] == 1 for q in p)
+    # in place: the plane pass writes the temporary, the z pass writes back
+    p, fc = describe(fl, [256, 256, 256], 1, aliased=True)
+    assert [(q["src"], q["dst"]) for q in p] == [(1, 2), (2, 1)] and not fc
+    # without plane fusion: three single passes (x contiguous, y and z strided)
+    p, fc = describe(fl, [256, 256, 256], 1, flags=4)
     assert [q["mode"] for q in p] == [0, 1, 1] and not fc and all(q["dst"] == 1 for q in p)
+    # double precision and split mode keep the separate passes
+    assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, prec=1)[0]] == [0, 1, 1]
+    assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, flags=2)[0]] == [0, 1, 1]
     # 1D 2^22 bit-reversed: four-step, strided 512 then contiguous 8192
     p, fc = describe(fl, [1 << 22], 1, 1)
     assert [(q["lg"], q["mode"], q["epi"]) for q in p] == [(9, 1, 1), (13, 0, 2)]
@@ -214,3 +236,120 @@ def test_bitrev_epilogue_runs(lg):
     got = RL * brev(j, lg - lgr) + brev(r, lgr)
     assert np.array_equal(got, brev(k, lg))
     assert np.unique(got).size == L
+
+
+# ---- fused 256 x 256 plane pass ------------------------------------------------
+
+def _cb_idx(row, col):
+    return row * 64 + (col ^ ((row & 7) << 2) ^ ((col >> 5) << 3))
+
+
+def _xg_idx(row, G):
+    return row * 32 + (G ^ (2 * (row & 3) + ((G >> 4) & 1)))
+
+
+def _gcol(c, q):
+    return 16 * (c >> 2) + 4 * q + (c & 3)
+
+
+def _dft16_quarter(v, g):
+    """dft16_head followed by the one-quarter output sum of fft_xy.hip."""
+    n2 = np.arange(4)
+    # head: Y[n1][k1] = sum_n2 v[4 n2 + n1] W4^{n2 k1}, then times W16^{n1 k1}
+    Y = np.zeros(v.shape[:-1] + (4, 4), complex)
+    for n1 in range(4):
+        for k1 in range(4):
+            Y[..., n1, k1] = (v[..., 4 * n2 + n1] * np.exp(-2j * np.pi * n2 * k1 / 4)).sum(-1)
+            Y[..., n1, k1] *= np.exp(-2j * np.pi * n1 * k1 / 16)
+    o = np.zeros(v.shape[:-1] + (4,), complex)
+    for k1 in range(4):
+        for n1 in range(4):
+            o[..., k1] += Y[..., n1, k1] * (-1j) ** ((n1 * g) % 4)
+    return o
+
+
+@pytest.mark.parametrize("g", range(4))
+def test_dft16_quarter(g):
+    rng = np.random.default_rng(g)
+    v = rng.standard_normal((5, 16)) + 1j * rng.standard_normal((5, 16))
+    assert np.allclose(_dft16_quarter(v, g), np.fft.fft(v, axis=1)[:, 4 * g:4 * g + 4], atol=1e-12)
+
+
+def emulate_plane256(plane):
+    """Replays k_fft_xy256's index maps for one plane (all four workgroups)."""
+    out = np.full((256, 256), np.nan + 0j)
+    tid = np.arange(1024)
+    w256 = lambda j: np.exp(-2j * np.pi * (j % 256) / 256)  # noqa: E731
+    r16 = np.arange(16)
+    for q in range(4):
+        # first row stage into 16-byte granules (pairs of complex)
+        gbuf = np.full((256 * 32, 2), np.nan + 0j)
+        t, rl = tid & 7, tid >> 3
+        for ch in range(2):
+            y = ch * 128 + rl
+            x0 = plane[y[:, None], 16 * r16[None, :] + 2 * t[:, None]]
+            x1 = plane[y[:, None], 16 * r16[None, :] + 2 * t[:, None] + 1]
+            o0, o1 = _dft16_quarter(x0, q), _dft16_quarter(x1, q)
+            for u in range(4):
+                g = _xg_idx(y, 8 * u + t)
+                assert np.isnan(gbuf[g]).all()
+                gbuf[g, 0], gbuf[g, 1] = o0[:, u], o1[:, u]
+        assert not np.isnan(gbuf).any()
+        # second row stage: lane (y, u), k1 = 4q + u
+        y, u = tid >> 2, tid & 3
+        v = np.zeros((1024, 16), complex)
+        for i in range(8):
+            g = _xg_idx(y, 8 * u + i)
+            v[:, 2 * i], v[:, 2 * i + 1] = gbuf[g, 0], gbuf[g, 1]
+        v = np.fft.fft(v * w256((4 * q + u)[:, None] * r16[None, :]), axis=1)
+        colbuf = np.full(256 * 64, np.nan + 0j)
+        for k2 in range(16):
+            idx = _cb_idx(y, 4 * k2 + u)
+            assert np.isnan(colbuf[idx]).all()
+            colbuf[idx] = v[:, k2]
+        assert not np.isnan(colbuf).any()
+        # column stage
+        c, tau = tid & 63, tid >> 6
+        v = colbuf[_cb_idx(tau[:, None] + 16 * r16[None, :], c[:, None])]
+        v = np.fft.fft(v, axis=1)
+        new = np.full(256 * 64, np.nan + 0j)
+        new[_cb_idx(16 * tau[:, None] + r16[None, :], c[:, None])] = v
+        assert not np.isnan(new).any()
+        v = new[_cb_idx(tau[:, None] + 16 * r16[None, :], c[:, None])]
+        v = np.fft.fft(v * w256(tau[:, None] * r16[None, :]), axis=1)
+        rows, cols = tau[:, None] + 16 * r16[None, :], _gcol(c, q)[:, None]
+        assert np.isnan(out[rows, cols]).all()
+        out[rows, cols] = v
+    return out
+
+
+def test_plane256_index_maps():
+    rng = np.random.default_rng(3)
+    plane = rng.standard_normal((256, 256)) + 1j * rng.standard_normal((256, 256))
+    got = emulate_plane256(plane)
+    assert not np.isnan(got).any()
+    assert np.allclose(got, np.fft.fft2(plane), rtol=1e-12, atol=1e-9)
+    # both swizzled LDS maps are bijections onto the buffer
+    rows, cols = np.meshgrid(np.arange(256), np.arange(64), indexing="ij")
+    idx = _cb_idx(rows, cols)
+    assert idx.min() == 0 and idx.max() == 256 * 64 - 1 and np.unique(idx).size == idx.size
+    rows, gr = np.meshgrid(np.arange(256), np.arange(32), indexing="ij")
+    idx = _xg_idx(rows, gr)
+    assert idx.min() == 0 and idx.max() == 256 * 32 - 1 and np.unique(idx).size == idx.size
+
+
+@pytest.mark.parametrize("nplanes", [1, 7, 8, 9, 256, 1000])
+def test_plane256_block_mapping(nplanes):
+    """Grid of ceil(P/8)*8*4 workgroups: block b is XCD b%8, quarter (b/8)%4 of
+    plane (b/32)*8 + b%8; every (plane, quarter) is taken exactly once."""
+    blocks = (nplanes + 7) // 8 * 8 * 4
+    b = np.arange(blocks)
+    xcd, k = b & 7, b >> 3
+    q, plane = k & 3, (k >> 2) * 8 + xcd
+    live = plane < nplanes
+    pairs = plane[live] * 4 + q[live]
+    assert np.array_equal(np.sort(pairs), np.arange(nplanes * 4))
+    # the four quarters of a plane share an XCD and are consecutive on it
+    for p in range(min(nplanes, 16)):
+        bs = b[live & (plane == p)]
+        assert len(set(bs & 7)) == 1 and np.array_equal(np.diff(np.sort(bs)), [8, 8, 8])
