@@ -61,8 +61,16 @@ def parse():
                     help="time the steps without per-kernel HIP events (diagnostic)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 10 plain device copies of one workload-sized array (counter calibration)")
+    ap.add_argument("--no-plane-fusion", action="store_true",
+                    help="run 256 x 256 planes as separate x and y passes (default: one fused pass)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the uneven-size batched vs plain per-transform comparison")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the split-precision 512^3 (spiked field) measurement")
+    ap.add_argument("--no-autograd", action="store_true",
+                    help="skip the differentiable 16 x 128^3 forward/backward measurement")
+    ap.add_argument("--no-cpu-fftw", action="store_true",
+                    help="skip the threaded FFTW-API CPU reference (MKL's fftwf wrappers)")
     return ap.parse_args()
 
 
@@ -82,6 +90,146 @@ def cpu_baseline(dims, seed):
             "kind": "port",
             "sample": f"{len(ts)} forward {'x'.join(map(str, dims))} transforms, FP64 oracle, "
                       f"median {t * 1e3:.1f} ms"}
+
+
+def cpu_fftwf(dims, seed):
+    """Threaded single-precision FFTW API on this host's cores: libfftw3f if
+    present, else MKL's FFTW3 wrappers (libmkl_rt).  Forward out-of-place
+    transform of the workload, plan built outside the timed region with
+    FFTW_ESTIMATE; median of 5.  Reported beside the oracle baseline."""
+    import numpy as np
+    from fftfpga_amd import datagen
+    lib = label = None
+    for name, lab in (("libfftw3f.so.3", "fftwf"), ("libfftw3f.so", "fftwf"),
+                      ("/opt/conda/lib/libmkl_rt.so", "fftwf API (MKL)"), ("libmkl_rt.so", "fftwf API (MKL)")):
+        try:
+            lib, label = ctypes.CDLL(name), lab
+            break
+        except OSError:
+            continue
+    if lib is None:
+        return {"available": False, "reason": "no libfftw3f or libmkl_rt on this host"}
+    nthreads = min(16, len(os.sched_getaffinity(0)))
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    plan_dft = lib.fftwf_plan_dft
+    plan_dft.restype = vp
+    plan_dft.argtypes = [ci, ctypes.POINTER(ci), vp, vp, ci, ctypes.c_uint]
+    lib.fftwf_execute.argtypes = [vp]
+    lib.fftwf_destroy_plan.argtypes = [vp]
+    threaded = False
+    if hasattr(lib, "fftwf_init_threads") and hasattr(lib, "fftwf_plan_with_nthreads"):
+        lib.fftwf_plan_with_nthreads.argtypes = [ci]
+        threaded = lib.fftwf_init_threads() != 0
+        if threaded:
+            lib.fftwf_plan_with_nthreads(nthreads)
+    n = math.prod(dims)
+    x = datagen.uniform(n, seed)
+    y = np.empty_like(x)
+    nd = (ci * len(dims))(*dims)
+    FFTW_FORWARD, FFTW_ESTIMATE = -1, 1 << 6
+    plan = plan_dft(len(dims), nd, x.ctypes.data, y.ctypes.data, FFTW_FORWARD, FFTW_ESTIMATE)
+    if not plan:
+        return {"available": False, "reason": f"{label}: plan creation failed"}
+    lib.fftwf_execute(plan)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        lib.fftwf_execute(plan)
+        ts.append(time.perf_counter() - t0)
+    lib.fftwf_destroy_plan(plan)
+    t = sorted(ts)[len(ts) // 2]
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": round(flops_per_transform(dims) / t / 1e9, 3), "unit": "GFLOP/s",
+            "threads": nthreads if threaded else 1, "lib": label, "plan": "FFTW_ESTIMATE",
+            "cpu": model, "sample": f"5 forward {'x'.join(map(str, dims))} transforms, median {t * 1e3:.2f} ms"}
+
+
+def split_512(steps):
+    """Split-precision mode at 512^3 on U[-1,1) plus +-1e-3 spikes on 0.1% of
+    the entries (splitmix64 seed 3): forward transform time of the FP32 path
+    and of the split path (radix-16 stages on FP16 matrix cores), and the
+    split result's relative L2 distance from the FP32 path.  (Accuracy against
+    the FP64 oracle is checked in the GPU tests.)"""
+    import torch
+
+    import fftfpga_amd as fl
+    from fftfpga_amd import datagen
+    N = 512
+    x = torch.from_numpy(datagen.uniform_with_spikes(N ** 3, 3)).cuda().view(N, N, N)
+    yp = torch.empty_like(x)
+    ys = torch.empty_like(x)
+
+    def run(split, out):
+        for _ in range(2):
+            fl.c2c_dev(x, 3, out=out, split=split)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fl.c2c_dev(x, 3, out=out, split=split)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    ms_p = run(False, yp)
+    ms_s = run(True, ys)
+    rel = (torch.linalg.vector_norm(ys - yp) / torch.linalg.vector_norm(yp)).item()
+    alg = 16.0 * N ** 3
+    fl_ = flops_per_transform([N, N, N])
+    del x, yp, ys
+    torch.cuda.empty_cache()
+    return {"workload": "512^3 C2C forward, U[-1,1) + +-1e-3 spikes on 0.1% of entries, seed 3",
+            "ms_fp32": round(ms_p, 4), "ms_split": round(ms_s, 4),
+            "gflops_fp32": round(fl_ / (ms_p * 1e-3) / 1e9, 1), "gflops_split": round(fl_ / (ms_s * 1e-3) / 1e9, 1),
+            "roofline_frac_fp32": round(alg / (ms_p * 1e-3) / HBM_PEAK, 4),
+            "roofline_frac_split": round(alg / (ms_s * 1e-3) / HBM_PEAK, 4),
+            "rel_l2_split_vs_fp32_path": rel}
+
+
+def autograd_16x128(steps):
+    """Differentiable transform y = W . FFT3(x) over a batch of 16 128^3
+    complex64 arrays (x ~ N(0,1), splitmix64 seed 4; W = |Re U[-1,1)| seed 5,
+    real 128^3): forward and hand-written adjoint backward (one weighted
+    inverse transform each), timed separately."""
+    import numpy as np
+    import torch
+
+    from fftfpga_amd import datagen
+    from fftfpga_amd.autograd import fft3
+    B, N = 16, 128
+    x = torch.from_numpy(datagen.gauss(B * N ** 3, 4).reshape(B, N, N, N)).cuda().requires_grad_(True)
+    w = torch.from_numpy(np.abs(datagen.uniform(N ** 3, 5).real).reshape(N, N, N).astype(np.float32)).cuda()
+    g = torch.from_numpy(datagen.gauss(B * N ** 3, 6).reshape(B, N, N, N)).cuda()
+    for _ in range(2):
+        y = fft3(x, w)
+        y.backward(g)
+    torch.cuda.synchronize()
+    tf = tb = 0.0
+    for _ in range(steps):
+        x.grad = None
+        t0 = time.perf_counter()
+        y = fft3(x, w)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        y.backward(g)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    ms_f, ms_b = tf * 1e3 / steps, tb * 1e3 / steps
+    alg_f, alg_b = 16.0 * B * N ** 3, 16.0 * B * N ** 3 + 4.0 * N ** 3
+    del x, w, g, y
+    torch.cuda.empty_cache()
+    return {"workload": "16 x 128^3, y = W . FFT3(x), x ~ N(0,1) seed 4, W = |Re U[-1,1)| seed 5",
+            "ms_forward": round(ms_f, 4), "ms_backward": round(ms_b, 4),
+            "hbm_gbs_forward": round(alg_f / (ms_f * 1e-3) / 1e9, 1),
+            "hbm_gbs_backward": round(alg_b / (ms_b * 1e-3) / 1e9, 1),
+            "roofline_frac_forward": round(alg_f / (ms_f * 1e-3) / HBM_PEAK, 4),
+            "roofline_frac_backward": round(alg_b / (ms_b * 1e-3) / HBM_PEAK, 4),
+            "timing": "host clock around each call with a device synchronisation (includes launch overhead)"}
 
 
 BATCH_COUNTS = {32: 256, 64: 160, 128: 32}
@@ -193,11 +341,13 @@ def main():
     lib = fl.load()
 
     bitrev = rank == 1
+    fusion = {"on": not args.no_plane_fusion}
+    fusable = rank >= 2 and dims[-1] == 256 and dims[-2] == 256
 
     def step():
-        fl.c2c_dev(x, rank, inverse=False, out=X, bitrev_1d=bitrev)
+        fl.c2c_dev(x, rank, inverse=False, out=X, bitrev_1d=bitrev, plane_fusion=fusion["on"])
         if round_trip:
-            fl.c2c_dev(X, rank, inverse=True, out=y, bitrev_1d=bitrev)
+            fl.c2c_dev(X, rank, inverse=True, out=y, bitrev_1d=bitrev, plane_fusion=fusion["on"])
 
     for _ in range(args.warmup):
         step()
@@ -263,6 +413,7 @@ def main():
         "data": f"synthetic: splitmix64 seed {seed}, re/im i.i.d. U[-1,1), complex64",
         "config": {"workload": args.workload, "dims": dims, "batch": batch,
                    "round_trip": round_trip, "placement": "out-of-place, HBM resident",
+                   "plane_fusion": fusable and fusion["on"],
                    "inverse": "unnormalised",
                    "output_order": "bit-reversed per transform" if bitrev else "natural"},
         "hbm_gbs_algorithmic": round(gbs, 1),
@@ -278,6 +429,15 @@ def main():
         "kernels": {k: {"count": v["count"], "avg_us": round(1e3 * v["total_ms"] / v["count"], 3)}
                     for k, v in kernels.items()},
     }
+    if fusable and fusion["on"]:
+        # the same K steps with the x and y axes as two separate passes
+        fusion["on"] = False
+        for _ in range(args.warmup):
+            step()
+        ms_unf, _ = timed(False)
+        fusion["on"] = True
+        result["unfused_passes"] = {"ms_per_step": round(ms_unf, 5),
+                                    "fusion_speedup": round(ms_unf / ms, 3)}
     if not args.no_compare:
         # the vendor library on the same device and data, for reference only
         for _ in range(3):
@@ -296,8 +456,16 @@ def main():
                                       "speedup_vs_torch": round(ms_t / ms, 3)}
     if not args.no_batched:
         result["batched"] = batched_vs_plain(max(5, min(args.steps, 20)), 2)
+    del x, X, y
+    torch.cuda.empty_cache()
+    if not args.no_split:
+        result["split_512"] = split_512(max(3, min(args.steps, 10)))
+    if not args.no_autograd:
+        result["autograd_16x128"] = autograd_16x128(max(3, min(args.steps, 10)))
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(dims, seed)
+    if not args.no_cpu_fftw:
+        result["cpu_fftwf"] = cpu_fftwf(dims, seed)
     print(json.dumps(result), flush=True)
 
 

@@ -27,7 +27,7 @@
 #include "engine.h"
 #include "fft_multi.h"
 #include "fft_pass.h"
-#include "fft_xy.h"
+#include "fft_plane.h"
 #include "fft_schedule.h"
 
 namespace fftamd {
@@ -99,7 +99,6 @@ struct State {
   size_t h_stage_cap = 0;
   std::map<PlanKey, Plan> plans;
   std::map<std::pair<int, int>, void*> tw_stockham;  // (prec, L)
-  void* tw_full256 = nullptr;                         // W_256^j, FP32
   std::map<std::pair<int, long long>, std::pair<void*, void*>> tw_fourstep;  // (prec, M)
 };
 
@@ -188,26 +187,6 @@ static int get_stockham(int prec, int L, const void** out) {
   return 0;
 }
 
-// W_256^j for j = 0..255 (FP32), for the fused plane pass
-static int get_full256(const void** out) {
-  State& st = S();
-  if (!st.tw_full256) {
-    std::vector<float> h(2 * 256);
-    for (int j = 0; j < 256; ++j) {
-      long double c, s;
-      twiddle_ld(j, 256, &c, &s);
-      h[2 * j] = (float)c;
-      h[2 * j + 1] = (float)s;
-    }
-    void* d = nullptr;
-    HIP_TRY(hipMalloc(&d, h.size() * sizeof(float)));
-    HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
-    st.tw_full256 = d;
-  }
-  *out = st.tw_full256;
-  return 0;
-}
-
 static unsigned fourstep_hb(long long M) { return (unsigned)((ilog2c(M) + 1) / 2); }
 
 template <typename R>
@@ -282,6 +261,10 @@ static int split_strided(int prec, int lg, int* lg1, int* lg2) {
   return 0;
 }
 
+// Fewest planes for which the one-workgroup-per-plane pass (planes up to
+// 128 x 128) replaces the separate x and y passes.
+static constexpr unsigned long long PLANE_MIN_COUNT = 1;
+
 static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec,
                       bool aliased, Plan& plan) {
   plan = Plan();
@@ -298,23 +281,29 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   const bool bitrev = (flags & FLAG_BITREV_1D) && rank == 1;
   std::vector<PassDesc> ps;
 
-  // 256 x 256 planes (the two fastest axes): one fused x+y pass
+  // planes of the two fastest axes in one fused x+y pass: 256 x 256 (four
+  // workgroups per plane, out of place) or up to 128 x 128 (one workgroup
+  // per plane in LDS, in place allowed)
   int top = rank - 1;
-  if (rank >= 2 && dims[rank - 1] == 256 && dims[rank - 2] == 256 && prec == PREC_F32 && !plan.split &&
-      !(flags & FLAG_NO_PLANE_FUSION)) {
+  if (rank >= 2 && prec == PREC_F32 && !plan.split && !(flags & FLAG_NO_PLANE_FUSION)) {
+    const long long LY = (long long)dims[rank - 2], LX = (long long)dims[rank - 1];
     unsigned long long O = batch;
     for (int d = 0; d < rank - 2; ++d) O *= dims[d];
-    PassDesc p;
-    p.lg = 8;
-    p.mode = MODE_PLANE256;
-    p.transposing = true;  // out of place: the four workgroups of a plane all read all of it
-    p.in_p = p.out_p = 1;
-    p.in_c = p.out_c = 256;
-    p.ncol = 256;
-    p.nouter = O;
-    p.in_o = p.out_o = 256 * 256;
-    ps.push_back(p);
-    top = rank - 3;
+    const bool big = LY == 256 && LX == 256;
+    if (big || (plane_shape_supported(LY, LX) && O >= PLANE_MIN_COUNT)) {
+      PassDesc p;
+      p.lg = ilog2c(LX);
+      p.mode = MODE_PLANE;
+      p.transposing = big;  // the four workgroups of a 256 x 256 plane all read all of it
+      p.in_p = p.out_p = 1;
+      p.in_c = p.out_c = LX;
+      p.ncol = (unsigned)LY;
+      p.K = (unsigned)LY;
+      p.nouter = O;
+      p.in_o = p.out_o = LY * LX;
+      ps.push_back(p);
+      top = rank - 3;
+    }
   }
 
   for (int a = top; a >= 0; --a) {
@@ -510,13 +499,15 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
   int cur = B_IN;
   for (size_t pi = 0; pi < plan.passes.size(); ++pi) {
     const PassDesc& p = plan.passes[pi];
-    if (p.mode == MODE_PLANE256) {
-      XYArgs xa{};
+    if (p.mode == MODE_PLANE) {
+      const int LX = 1 << p.lg, LY = (int)p.K;
+      PlaneArgs xa{};
       xa.in = ptr(p.src);
       xa.out = ptr(p.dst);
       xa.nplanes = p.nouter;
       xa.plane_stride = p.in_o;
-      if (get_stockham(plan.prec, 256, &xa.tw) || get_full256(&xa.tw_full)) return -1;
+      if (sched_np(LX) > 1 && get_stockham(plan.prec, LX, &xa.tw_x)) return -1;
+      if (sched_np(LY) > 1 && get_stockham(plan.prec, LY, &xa.tw_y)) return -1;
       xa.w_load = pi == 0 ? w_load : nullptr;
       xa.w_store = pi + 1 == plan.passes.size() ? w_store : nullptr;
       xa.w_mask = (unsigned long long)plan.per - 1ULL;
@@ -524,10 +515,12 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       Prof& pf = P();
       const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
-      const int rc = launch_fft_xy256(xa, stream);
+      const int rc = (LY == 256 && LX == 256) ? launch_fft_xy256(xa, stream) : launch_fft_plane(LY, LX, xa, stream);
       if (prof) {
         HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
-        pf.recs.push_back({"fft_plane_f32_256x256", pf.pool[pf.used], pf.pool[pf.used + 1]});
+        char lab[64];
+        snprintf(lab, sizeof(lab), "fft_plane_f32_%dx%d", LY, LX);
+        pf.recs.push_back({lab, pf.pool[pf.used], pf.pool[pf.used + 1]});
         pf.used += 2;
       }
       if (rc) {
@@ -984,8 +977,6 @@ static void release_locked() {
     (void)hipFree(kv.second.second);
   }
   st.tw_fourstep.clear();
-  if (st.tw_full256) (void)hipFree(st.tw_full256);
-  st.tw_full256 = nullptr;
   if (st.d_in) (void)hipFree(st.d_in);
   if (st.d_out) (void)hipFree(st.d_out);
   if (st.h_stage) (void)hipHostFree(st.h_stage);
@@ -1123,8 +1114,9 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
     long long v[NF] = {d.lg, d.mode, d.epi, d.transposing, d.in_p, d.out_p, d.in_c, d.out_c,
                        d.in_k, d.out_k, d.in_o, d.out_o, d.ncol, d.K, d.kbits,
                        (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
-                       d.mode == MODE_PLANE256 ? 64 : geom_cols(1 << d.lg, d.mode),
-                       d.mode == MODE_PLANE256 ? 1024 : geom_threads(1 << d.lg, d.mode)};
+                       d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K) : geom_cols(1 << d.lg, d.mode),
+                       d.mode == MODE_PLANE ? (d.K == 256 ? 1024 : ((long long)d.K << d.lg) / 16)
+                                            : geom_threads(1 << d.lg, d.mode)};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }
   out[NF * n] = p.final_copy ? 1 : 0;

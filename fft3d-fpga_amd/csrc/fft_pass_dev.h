@@ -21,11 +21,13 @@
 
 namespace fftamd {
 
-template <int L, int MODE>
+// CC: lines per workgroup (the pass kernels' default; the plane kernels hold
+// a whole plane's rows or columns)
+template <int L, int MODE, int CC = geom_cols(L, MODE)>
 struct Geom {
   static constexpr int PPT = geom_ppt(L);
   static constexpr int TPL = geom_tpl(L);
-  static constexpr int C = geom_cols(L, MODE);
+  static constexpr int C = CC;
   static constexpr int NT = C * TPL;
   static constexpr int NP = sched_np(L);
   // contiguous: [line][p + p/16]; strided: [p][col] with one pad column
@@ -40,10 +42,10 @@ struct Geom {
   }
 };
 
-template <typename R, int L, int MODE, int S, bool SPLIT = false>
+template <typename R, int L, int MODE, int S, bool SPLIT = false, int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* __restrict__ tw,
                                               cpx<R>* lds = nullptr) {
-  using G = Geom<L, MODE>;
+  using G = Geom<L, MODE, CC>;
   constexpr int RAD = sched_radix(L, S);
   constexpr int NS = sched_ns(L, S);
   constexpr int NB = G::PPT / RAD;
@@ -91,9 +93,9 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* 
 }
 
 // write stage-S results to LDS at their Stockham destinations
-template <typename R, int L, int MODE, int S>
+template <typename R, int L, int MODE, int S, int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void stage_store_lds(const cpx<R>* v, cpx<R>* lds, int c, int tau) {
-  using G = Geom<L, MODE>;
+  using G = Geom<L, MODE, CC>;
   constexpr int RAD = sched_radix(L, S);
   constexpr int NS = sched_ns(L, S);
   constexpr int NB = G::PPT / RAD;
@@ -107,9 +109,9 @@ __device__ __forceinline__ void stage_store_lds(const cpx<R>* v, cpx<R>* lds, in
 }
 
 // read stage-S butterfly inputs from LDS
-template <typename R, int L, int MODE, int S>
+template <typename R, int L, int MODE, int S, int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void stage_load_lds(cpx<R>* v, const cpx<R>* lds, int c, int tau) {
-  using G = Geom<L, MODE>;
+  using G = Geom<L, MODE, CC>;
   constexpr int RAD = sched_radix(L, S);
   constexpr int NB = G::PPT / RAD;
 #pragma unroll
@@ -120,21 +122,21 @@ __device__ __forceinline__ void stage_load_lds(cpx<R>* v, const cpx<R>* lds, int
   }
 }
 
-template <typename R, int L, int MODE, int S, bool SPLIT = false>
+template <typename R, int L, int MODE, int S, bool SPLIT = false, int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void run_stages(cpx<R>* v, cpx<R>* lds, int c, int tau,
                                            const cpx<R>* __restrict__ tw) {
-  using G = Geom<L, MODE>;
+  using G = Geom<L, MODE, CC>;
   if constexpr (S < G::NP) {
     if constexpr (S > 0) {
       __syncthreads();  // previous stage's LDS writes are visible
-      stage_load_lds<R, L, MODE, S>(v, lds, c, tau);
+      stage_load_lds<R, L, MODE, S, CC>(v, lds, c, tau);
     }
-    stage_compute<R, L, MODE, S, SPLIT>(v, tau, tw, lds);
+    stage_compute<R, L, MODE, S, SPLIT, CC>(v, tau, tw, lds);
     if constexpr (S + 1 < G::NP) {
       constexpr bool split_stage = SPLIT && sched_radix(L, S) == 16;
       if constexpr (S > 0 && !split_stage) __syncthreads();  // every thread finished reading this stage
-      stage_store_lds<R, L, MODE, S>(v, lds, c, tau);
-      run_stages<R, L, MODE, S + 1, SPLIT>(v, lds, c, tau, tw);
+      stage_store_lds<R, L, MODE, S, CC>(v, lds, c, tau);
+      run_stages<R, L, MODE, S + 1, SPLIT, CC>(v, lds, c, tau, tw);
     }
   }
 }
@@ -189,9 +191,12 @@ __device__ __forceinline__ void tile_load(const PassArgs& a, const TileLoc& t, c
 }
 
 // Input weights and direction, radix stages and the epilogue of one tile
-// whose raw inputs are in v.
-template <typename R, int L, int MODE, int EPI, bool SPLIT = false>
+// whose raw inputs are in v.  DIR: 0 forward, 1 inverse (the swaps of
+// swap(DFT(swap(x))) are register renames), -1 taken from a.inverse at run
+// time (per-element selects).
+template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1>
 __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t, cpx<R>* v, cpx<R>* lds) {
+  const bool inv = DIR < 0 ? a.inverse != 0 : DIR == 1;
   using G = Geom<L, MODE>;
   constexpr int PPT = G::PPT, TPL = G::TPL;
   constexpr int NP = G::NP;
@@ -214,7 +219,7 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
           const long long p = j + r * (L / RAD);
           x = cscale(x, (R)a.w_load[(unsigned long long)(in_off + p * a.in_p) & a.w_mask]);
         }
-        v[b * RAD + r] = a.inverse ? cswap(x) : x;
+        v[b * RAD + r] = inv ? cswap(x) : x;
       }
     }
   }
@@ -238,7 +243,7 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
       for (int r = 0; r < RL; ++r) {
         const long long q = run + brev_bits((unsigned)r, LGR);
         cpx<R> x = v[b * RL + r];
-        if (a.inverse) x = cswap(x);
+        if (inv) x = cswap(x);
         if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + q * a.out_p) & a.w_mask]);
         if (active) out[q * a.out_p] = x;
       }
@@ -258,7 +263,7 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
           const cpx<R> wh = reinterpret_cast<const cpx<R>*>(a.ptw_hi)[t >> a.ptw_hb];
           x = cmul(x, cmul(wh, wl));
         }
-        if (a.inverse) x = cswap(x);
+        if (inv) x = cswap(x);
         if (active) {
           if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + (long long)k * a.out_p) & a.w_mask]);
           out[(long long)k * a.out_p] = x;
@@ -270,12 +275,12 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
 
 // One workgroup tile of a pass (block index g).  Used by k_fft_pass (one tile
 // per workgroup) and by the persistent multi-transform kernel.
-template <typename R, int L, int MODE, int EPI, bool SPLIT = false>
+template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1>
 __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds) {
   cpx<R> v[Geom<L, MODE>::PPT];
   const TileLoc t = tile_locate<L, MODE>(a, g);
   tile_load<R, L, MODE>(a, t, v);
-  tile_finish<R, L, MODE, EPI, SPLIT>(a, t, v, lds);
+  tile_finish<R, L, MODE, EPI, SPLIT, DIR>(a, t, v, lds);
 }
 
 }  // namespace fftamd

@@ -1,0 +1,31 @@
+// fft_plane.h -- arguments of the plane passes: the 2D transform of every
+// (LY x LX) plane of an array, both axes in one pass over HBM.
+#pragma once
+
+namespace fftamd {
+
+struct PlaneArgs {
+  const void* in;
+  void* out;
+  unsigned long long nplanes;    // planes in the array
+  long long plane_stride;        // complex elements between planes
+  const void* tw_x;              // Stockham twiddles of the LX-point rows
+  const void* tw_y;              // Stockham twiddles of the LY-point columns
+  const float* w_load;           // optional real weights (see PassArgs)
+  const float* w_store;
+  unsigned long long w_mask;
+  int inverse;
+};
+
+// 256 x 256 planes, four workgroups per plane (fft_xy.hip).  0 on success, -1
+// when the grid does not fit, -2 on a launch error.  Out of place only: in
+// and out must not overlap.
+int launch_fft_xy256(const PlaneArgs& a, void* stream);
+
+// LY x LX planes with 16 <= LY, LX <= 128 and LY * LX >= 1024, one workgroup
+// per plane holding it in LDS (fft_plane.hip).  0 on success, -1 for an
+// unsupported shape or grid, -2 on a launch error.  In place allowed.
+int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream);
+bool plane_shape_supported(long long ly, long long lx);
+
+}  // namespace fftamd

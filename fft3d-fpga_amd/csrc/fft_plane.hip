@@ -1,0 +1,137 @@
+// fft_plane.hip -- the 2D transform of every LY x LX plane of an array in one
+// pass over HBM, for planes that fit one workgroup's LDS (LY, LX <= 128).
+//
+// One workgroup per plane, LY * LX / 16 threads, 16 points per thread in both
+// phases.  Rows: each thread loads its first-stage butterfly inputs of one row
+// straight from HBM (coalesced along the row), runs the Stockham stages with
+// LDS exchanges ([row][x + x/16] layout), and writes the row's results into
+// the column layout ([y][col], one pad column).  Columns: the same stages
+// along y, and the last stage writes natural-order rows to HBM (coalesced
+// along x).  A 3D transform of 128^3-sized planes then takes two passes over
+// HBM instead of three, and the plane is read and written exactly once.
+//
+// The radix stages are the line passes' (fft_pass_dev.h), instantiated with a
+// whole plane's rows or columns per workgroup.
+#include <hip/hip_runtime.h>
+
+#include "fft_pass_dev.h"
+#include "fft_plane.h"
+
+namespace fftamd {
+
+template <int LY, int LX>
+struct PlaneGeom {
+  static_assert(LY >= 16 && LX >= 16 && LY <= 128 && LX <= 128, "plane sides 16..128");
+  using GX = Geom<LX, MODE_CONTIG, LY>;    // rows: LY lines of LX points
+  using GY = Geom<LY, MODE_STRIDED, LX>;   // columns: LX lines of LY points
+  static constexpr int NT = LY * LX / 16;
+  static_assert(GX::NT == NT && GY::NT == NT, "16 points per thread in both phases");
+  static constexpr int LDS = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
+};
+
+template <int LY, int LX, bool INV>
+__global__ void __launch_bounds__(LY * LX / 16) k_fft_plane(const PlaneArgs a) {
+  using PG = PlaneGeom<LY, LX>;
+  using GX = typename PG::GX;
+  using GY = typename PG::GY;
+  __shared__ cpx<float> lds[PG::LDS];
+  const unsigned long long plane = blockIdx.x;
+  const long long base = (long long)plane * a.plane_stride;
+  const cpx<float>* __restrict__ in = reinterpret_cast<const cpx<float>*>(a.in) + base;
+  cpx<float>* __restrict__ out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  const int tid = threadIdx.x;
+  cpx<float> v[16];
+
+  // ---- rows: thread (row c, tau); stage 0 reads its butterfly inputs from HBM
+  {
+    constexpr int RAD = sched_radix(LX, 0), NB = 16 / RAD;
+    const int c = tid / GX::TPL, tau = tid % GX::TPL;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = tau + b * GX::TPL;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        const int e = c * LX + j + r * (LX / RAD);
+        cpx<float> x = in[e];
+        if (a.w_load) x = cscale(x, a.w_load[(unsigned long long)(base + e) & a.w_mask]);
+        v[b * RAD + r] = INV ? cswap(x) : x;
+      }
+    }
+    run_stages<float, LX, MODE_CONTIG, 0, false, LY>(v, lds, c, tau,
+                                                     reinterpret_cast<const cpx<float>*>(a.tw_x));
+    // last stage: point k = j + r*NS of row c -> column layout
+    constexpr int NPX = GX::NP;
+    constexpr int RL = sched_radix(LX, NPX - 1), NSL = sched_ns(LX, NPX - 1), NBL = 16 / RL;
+    __syncthreads();  // every row stage has read its LDS inputs
+#pragma unroll
+    for (int b = 0; b < NBL; ++b) {
+      const int j = tau + b * GX::TPL;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) lds[GY::idx(j + r * NSL, c)] = v[b * RL + r];
+    }
+  }
+  __syncthreads();
+
+  // ---- columns: thread (column c, tau), adjacent threads on adjacent columns
+  {
+    const int c = tid % LX, tau = tid / LX;
+    stage_load_lds<float, LY, MODE_STRIDED, 0, LX>(v, lds, c, tau);
+    __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
+    run_stages<float, LY, MODE_STRIDED, 0, false, LX>(v, lds, c, tau,
+                                                      reinterpret_cast<const cpx<float>*>(a.tw_y));
+    constexpr int NPY = GY::NP;
+    constexpr int RL = sched_radix(LY, NPY - 1), NSL = sched_ns(LY, NPY - 1), NBL = 16 / RL;
+#pragma unroll
+    for (int b = 0; b < NBL; ++b) {
+      const int j = tau + b * GY::TPL;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int e = (j + r * NSL) * LX + c;
+        cpx<float> x = v[b * RL + r];
+        if (INV) x = cswap(x);
+        if (a.w_store) x = cscale(x, a.w_store[(unsigned long long)(base + e) & a.w_mask]);
+        out[e] = x;
+      }
+    }
+  }
+}
+
+bool plane_shape_supported(long long ly, long long lx) {
+  auto p2 = [](long long v) { return v >= 16 && v <= 128 && (v & (v - 1)) == 0; };
+  return p2(ly) && p2(lx) && ly * lx >= 1024;
+}
+
+template <int LY, int LX>
+static void launch_shape(const PlaneArgs& a, hipStream_t s) {
+  constexpr int NT = PlaneGeom<LY, LX>::NT;
+  if (a.inverse) hipLaunchKernelGGL((k_fft_plane<LY, LX, true>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+  else hipLaunchKernelGGL((k_fft_plane<LY, LX, false>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+}
+
+template <int LY>
+static bool dispatch_lx(int lx, const PlaneArgs& a, hipStream_t s) {
+  switch (lx) {
+    case 16: if constexpr (LY * 16 >= 1024) { launch_shape<LY, 16>(a, s); return true; } else return false;
+    case 32: if constexpr (LY * 32 >= 1024) { launch_shape<LY, 32>(a, s); return true; } else return false;
+    case 64: launch_shape<LY, 64>(a, s); return true;
+    case 128: launch_shape<LY, 128>(a, s); return true;
+    default: return false;
+  }
+}
+
+int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!plane_shape_supported(ly, lx) || a.nplanes == 0 || a.nplanes > 0xFFFFFFFFull / 1024) return -1;
+  bool ok = false;
+  switch (ly) {
+    case 16: ok = dispatch_lx<16>(lx, a, s); break;
+    case 32: ok = dispatch_lx<32>(lx, a, s); break;
+    case 64: ok = dispatch_lx<64>(lx, a, s); break;
+    case 128: ok = dispatch_lx<128>(lx, a, s); break;
+    default: break;
+  }
+  if (!ok) return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace fftamd

@@ -11,8 +11,9 @@
 
 namespace fftamd {
 
-// MODE_PLANE256: fused x+y pass over 256 x 256 planes (fft_xy.hip)
-enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1, MODE_PLANE256 = 2 };
+// MODE_PLANE: fused x+y pass over whole planes (fft_xy.hip for 256 x 256,
+// fft_plane.hip up to 128 x 128)
+enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1, MODE_PLANE = 2 };
 // EPI_TWIDDLE multiplies point k of column c by W_M^{(c >> cshift) * k}
 // (four-step split of a long axis); EPI_BITREV stores point k at bitrev(k).
 enum PassEpilogue { EPI_NONE = 0, EPI_TWIDDLE = 1, EPI_BITREV = 2 };

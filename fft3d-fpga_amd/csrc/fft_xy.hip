@@ -20,7 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fft_pass_dev.h"
-#include "fft_xy.h"
+#include "fft_plane.h"
 
 namespace fftamd {
 
@@ -106,7 +106,7 @@ __device__ __forceinline__ void load_rows(const cpx<float>* __restrict__ in, int
 // First stage for one chunk of rows: the 16-point DFTs over n2 of n1 = 2t
 // and 2t + 1, four outputs k1 = 4Q + u each, into the LDS granules.
 template <int Q, bool INV>
-__device__ __forceinline__ void first_stage(const XYArgs& a, const float4* p, unsigned long long plane, int y,
+__device__ __forceinline__ void first_stage(const PlaneArgs& a, const float4* p, unsigned long long plane, int y,
                                             int t, float4* gbuf) {
   cpx<float> x0[16], x1[16];
 #pragma unroll
@@ -132,7 +132,7 @@ __device__ __forceinline__ void first_stage(const XYArgs& a, const float4* p, un
 // columns in buf (column layout).  INV: inverse transform as
 // swap(DFT(swap(x))), the swaps being register renames.
 template <int Q, bool INV>
-__device__ __forceinline__ void row_stages(const XYArgs& a, const cpx<float>* __restrict__ in,
+__device__ __forceinline__ void row_stages(const PlaneArgs& a, const cpx<float>* __restrict__ in,
                                            unsigned long long plane, cpx<float>* buf) {
   const int tid = threadIdx.x;
   float4* gbuf = reinterpret_cast<float4*>(buf);
@@ -156,7 +156,7 @@ __device__ __forceinline__ void row_stages(const XYArgs& a, const cpx<float>* __
     v[2 * i] = {p.x, p.y};
     v[2 * i + 1] = {p.z, p.w};
   }
-  twiddle256(v, 4 * Q + u, reinterpret_cast<const cpx<float>*>(a.tw));
+  twiddle256(v, 4 * Q + u, reinterpret_cast<const cpx<float>*>(a.tw_x));
   dft16(v);
   __syncthreads();  // every first-stage granule has been read
 #pragma unroll
@@ -164,7 +164,7 @@ __device__ __forceinline__ void row_stages(const XYArgs& a, const cpx<float>* __
 }
 
 template <bool INV>
-__global__ void __launch_bounds__(XY_NT) k_fft_xy256(const XYArgs a) {
+__global__ void __launch_bounds__(XY_NT) k_fft_xy256(const PlaneArgs a) {
   __shared__ __align__(16) cpx<float> buf[XY_L * XY_QC];
   // XCD-aware mapping: block b runs on XCD b % 8 (dispatch is round-robin
   // over XCDs); the k-th block of an XCD takes quarter k % 4 of plane
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(XY_NT) k_fft_xy256(const XYArgs a) {
   if (plane >= a.nplanes) return;  // uniform over the workgroup
   const cpx<float>* __restrict__ in = reinterpret_cast<const cpx<float>*>(a.in) + plane * a.plane_stride;
   cpx<float>* __restrict__ out = reinterpret_cast<cpx<float>*>(a.out) + plane * a.plane_stride;
-  const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
+  const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw_x);
   const int tid = threadIdx.x;
 
   switch (q) {  // uniform: the quarter is a compile-time constant inside
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(XY_NT) k_fft_xy256(const XYArgs a) {
   }
 }
 
-int launch_fft_xy256(const XYArgs& a, void* stream) {
+int launch_fft_xy256(const PlaneArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   // 4 blocks per plane, planes rounded up to a multiple of 8 (one per XCD)
   const unsigned long long planes8 = (a.nplanes + 7ull) / 8ull * 8ull;

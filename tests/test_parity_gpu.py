@@ -304,10 +304,14 @@ def test_device_api_orders_with_torch_default_stream(gpu_lib):
 # ---- fused 256 x 256 plane pass ------------------------------------------------------------------
 
 @pytest.mark.parametrize("shape,rank", [((1, 256, 256), 2), ((9, 256, 256), 2), ((4, 256, 256), 3),
-                                        ((2, 8, 256, 256), 3)])
+                                        ((2, 8, 256, 256), 3), ((1, 128, 128), 2), ((3, 64, 128), 2),
+                                        ((2, 128, 64), 3), ((16, 32, 32), 3), ((5, 16, 64), 2),
+                                        ((128, 128, 128), 3), ((3, 64, 64, 64), 3)])
 def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
-    """Plane counts below, at and above one per XCD slot (1, 4, 9, 16); out of
-    place and in place; against the FP64 oracle and the unfused three-pass path."""
+    """256 x 256 planes: counts below, at and above one per XCD slot (1, 4, 9,
+    16); planes up to 128 x 128 (one workgroup each): square and rectangular
+    shapes.  Out of place and in place; against the FP64 oracle and the
+    unfused path."""
     import torch
     from fftfpga_amd import c2c_dev
     n = int(np.prod(shape))
@@ -330,7 +334,8 @@ def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
     assert np.array_equal(xt.cpu().numpy().reshape(-1), x)  # input untouched
 
 
-@pytest.mark.parametrize("shape,rank", [((2, 256, 256), 2), ((4, 256, 256), 3)])
+@pytest.mark.parametrize("shape,rank", [((2, 256, 256), 2), ((4, 256, 256), 3), ((2, 128, 128), 2),
+                                        ((4, 64, 64), 3)])
 def test_plane_pass_weights(gpu_lib, orc, shape, rank):
     """Weights applied in the fused pass's loads (both ranks) and stores (rank 2)."""
     import torch

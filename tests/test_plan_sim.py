@@ -115,11 +115,14 @@ def run_plan(passes, final_copy, x, inverse, aliased=False):
     for p in passes:
         L = 1 << p["lg"]
         C = p["C"]
-        if p["mode"] == 2:  # fused 256 x 256 plane pass, out of place
-            assert p["src"] != p["dst"] and p["transposing"]
-            assert p["nouter"] * p["in_o"] == total and p["in_o"] == 65536
-            planes = bufs[p["src"]].reshape(p["nouter"], 256, 256)
-            X = np.fft.ifft2(planes) * 65536 if inverse else np.fft.fft2(planes)
+        if p["mode"] == 2:  # fused plane pass: 2D transform of every (K x 2^lg) plane
+            LY, LX = p["K"], 1 << p["lg"]
+            assert p["transposing"] == (LY == 256 and LX == 256)  # the 256 x 256 pass runs out of place
+            if p["transposing"]:
+                assert p["src"] != p["dst"]
+            assert p["nouter"] * p["in_o"] == total and p["in_o"] == LY * LX
+            planes = bufs[p["src"]].reshape(p["nouter"], LY, LX)
+            X = np.fft.ifft2(planes) * (LY * LX) if inverse else np.fft.fft2(planes)
             bufs[p["dst"]][:] = X.reshape(-1)
             cur = p["dst"]
             continue
@@ -167,7 +170,8 @@ CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16]
          ([1024, 8], 1, 0), ([2048, 32], 1, 0), ([8, 4096], 1, 0), ([1, 1, 1], 2, 0),
          ([2, 1, 4], 1, 0), ([256], 5, 1), ([1 << 14], 2, 1), ([1 << 14], 2, 0), ([1 << 16], 1, 1),
          ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0), ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
-         ([256, 256], 1, 4), ([2, 256, 256], 1, 4)]
+         ([256, 256], 1, 4), ([2, 256, 256], 1, 4), ([4, 128, 128], 1, 0), ([128, 64], 3, 0),
+         ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4)]
 
 
 @pytest.mark.parametrize("dims,batch,flags", CASES)
