@@ -604,6 +604,13 @@ static double ms_between(hipEvent_t a, hipEvent_t b);
 // ---------------------------------------------------------------------------
 // multi-transform plans: transforms of uneven sizes in one persistent launch
 // ---------------------------------------------------------------------------
+// Transforms of one shape whose x and y passes run as the plane kernel (mode 2).
+struct PlaneGroup {
+  int ly = 0, lx = 0;
+  unsigned ppt = 0, count = 0;  // planes per transform, transforms
+  size_t tab_off = 0;           // first (in, out) pair in the device table
+};
+
 struct MultiPlan {
   std::vector<MultiPass> passes;
   std::vector<MultiItem> items;
@@ -614,8 +621,16 @@ struct MultiPlan {
   unsigned* d_phase = nullptr;
   MultiCtl* d_ctl = nullptr;
   unsigned grid = 0, chunk = 4;
-  int mode = 0;  // 0: one launch, grid barriers between phases; 1: one launch per phase
+  // 0: one launch, grid barriers between phases; 1: one launch per phase;
+  // 2: x and y of each shape group as plane kernels, then the z phase
+  int mode = 0;
   bool split = false;
+  bool plane_ok = false;  // mode 2 available (every transform 3D, planes up to 128 x 128)
+  std::vector<PlaneGroup> groups;
+  unsigned long long* d_tab = nullptr;
+  const void* din = nullptr;
+  void* dout = nullptr;
+  bool inverse = false;
   std::string sig;
   size_t points = 0;
   ~MultiPlan() {
@@ -623,6 +638,7 @@ struct MultiPlan {
     if (d_items) (void)hipFree(d_items);
     if (d_phase) (void)hipFree(d_phase);
     if (d_ctl) (void)hipFree(d_ctl);
+    if (d_tab) (void)hipFree(d_tab);
   }
 };
 
@@ -669,7 +685,13 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
       }
     Plan p;
     const bool aliased = (const char*)din + 8 * in_off[i] == (const char*)dout + 8 * out_off[i];
-    if (build_plan(3, d, 1, 0u, PREC_F32, aliased, p)) return -1;
+    // line passes only: the persistent kernel's tiles are 256-thread line tiles
+    if (build_plan(3, d, 1, FLAG_NO_PLANE_FUSION, PREC_F32, aliased, p)) return -1;
+    for (const PassDesc& q : p.passes)
+      if (q.mode != MODE_CONTIG && q.mode != MODE_STRIDED) {
+        set_error("multi-transform plan: unexpected pass kind");
+        return -1;
+      }
     if (p.needs_tmp || p.final_copy) {
       set_error("multi-transform plan needs a temporary buffer");
       return -1;
@@ -751,6 +773,36 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
     HIP_TRY(hipMemcpy(mp->d_items, mp->items.data(), sizeof(MultiItem) * mp->items.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(mp->d_phase, mp->phase_start.data(), sizeof(unsigned) * mp->phase_start.size(),
                     hipMemcpyHostToDevice));
+  // mode 2: every transform has x, y and z passes and planes the plane kernel holds
+  mp->din = din;
+  mp->dout = dout;
+  mp->inverse = inverse;
+  mp->plane_ok = !mp->split && nphase == 3;
+  for (unsigned i = 0; i < count && mp->plane_ok; ++i)
+    mp->plane_ok = trs[i].ps.size() == 3 && plane_shape_supported(dims3[3 * i + 1], dims3[3 * i + 2]);
+  if (mp->plane_ok) {
+    std::map<std::tuple<unsigned, unsigned, unsigned>, std::vector<unsigned>> by_shape;
+    for (unsigned oi = 0; oi < count; ++oi) {
+      const unsigned i = order[oi];
+      by_shape[std::make_tuple(dims3[3 * i], dims3[3 * i + 1], dims3[3 * i + 2])].push_back(i);
+    }
+    std::vector<unsigned long long> tab;
+    for (auto it = by_shape.rbegin(); it != by_shape.rend(); ++it) {  // largest shapes first
+      PlaneGroup g;
+      g.ppt = std::get<0>(it->first);
+      g.ly = (int)std::get<1>(it->first);
+      g.lx = (int)std::get<2>(it->first);
+      g.count = (unsigned)it->second.size();
+      g.tab_off = tab.size() / 2;
+      for (unsigned i : it->second) {
+        tab.push_back(in_off[i]);
+        tab.push_back(out_off[i]);
+      }
+      mp->groups.push_back(g);
+    }
+    HIP_TRY(hipMalloc(&mp->d_tab, sizeof(unsigned long long) * tab.size()));
+    HIP_TRY(hipMemcpy(mp->d_tab, tab.data(), sizeof(unsigned long long) * tab.size(), hipMemcpyHostToDevice));
+  }
   unsigned per = 0;
   if (multi_default_grid(&mp->grid, &per, mp->split)) return -1;
   *handle = mp.release();
@@ -764,6 +816,30 @@ int multi_exec(void* handle, void* stream) {
     return -1;
   }
   if (mp->nphase == 0) return 0;
+  if (mp->mode == 2) {
+    for (const PlaneGroup& g : mp->groups) {
+      PlaneArgs a{};
+      a.in = mp->din;
+      a.out = mp->dout;
+      a.nplanes = (unsigned long long)g.count * g.ppt;
+      a.plane_stride = (long long)g.ly * g.lx;
+      if (sched_np(g.lx) > 1 && get_stockham(PREC_F32, g.lx, &a.tw_x)) return -1;
+      if (sched_np(g.ly) > 1 && get_stockham(PREC_F32, g.ly, &a.tw_y)) return -1;
+      a.inverse = mp->inverse ? 1 : 0;
+      a.tab = mp->d_tab + 2 * g.tab_off;
+      a.ppt = g.ppt;
+      if (launch_fft_plane(g.ly, g.lx, a, stream)) {
+        set_error("plane kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+        return -1;
+      }
+    }
+    if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, 2, 1, mp->d_ctl, mp->grid, mp->chunk,
+                     g_multi_spin_limit, stream, mp->split)) {
+      set_error("multi-transform launch failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+    return 0;
+  }
   if (mp->mode == 0) {
     if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, 0, mp->nphase, mp->d_ctl, mp->grid, mp->chunk,
                      g_multi_spin_limit, stream, mp->split)) {
@@ -815,8 +891,9 @@ int multi_set_schedule(void* handle, int mode, unsigned grid, unsigned chunk) {
   MultiPlan* mp = static_cast<MultiPlan*>(handle);
   unsigned g = 0, per = 0;
   if (!mp || multi_default_grid(&g, &per, mp->split)) return -1;
-  if ((mode != 0 && mode != 1) || chunk == 0 || grid == 0 || grid > g) {
-    set_error("schedule out of range (grid must be 1..%u resident workgroups)", g);
+  if (mode < 0 || mode > 2 || (mode == 2 && !mp->plane_ok) || chunk == 0 || grid == 0 || grid > g) {
+    set_error("schedule out of range (modes 0, 1%s; grid must be 1..%u resident workgroups)",
+              mp->plane_ok ? ", 2" : "", g);
     return -1;
   }
   mp->mode = mode;
@@ -848,7 +925,8 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
       int mode;
       unsigned pc, ch;
       double us;
-      if ((ls >> sg >> mode >> pc >> ch >> us) && sg == mp->sig && pc >= 1 && pc <= per) {
+      if ((ls >> sg >> mode >> pc >> ch >> us) && sg == mp->sig && pc >= 1 && pc <= per && mode >= 0 &&
+          mode <= (mp->plane_ok ? 2 : 1)) {
         mp->mode = mode;
         mp->grid = pc * ncu;
         mp->chunk = ch;
@@ -865,7 +943,7 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
   int bmode = 0;
   unsigned bpc = per, bch = 4;
   const unsigned chunks[] = {1, 2, 4, 8, 16};
-  for (int mode = 0; mode <= 1; ++mode)
+  for (int mode = 0; mode <= (mp->plane_ok ? 2 : 1); ++mode)
     for (unsigned pc = 1; pc <= per; ++pc)
       for (unsigned ch : chunks) {
         mp->mode = mode;

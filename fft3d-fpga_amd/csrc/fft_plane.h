@@ -15,6 +15,12 @@ struct PlaneArgs {
   const float* w_store;
   unsigned long long w_mask;
   int inverse;
+  // optional transform table (planes up to 128 x 128 only): plane b belongs
+  // to transform b / ppt, whose first input and output elements are
+  // tab[2 t] and tab[2 t + 1] (complex elements from in / out); NULL: plane b
+  // starts at b * plane_stride in both
+  const unsigned long long* tab;
+  unsigned ppt;
 };
 
 // 256 x 256 planes, four workgroups per plane (fft_xy.hip).  0 on success, -1

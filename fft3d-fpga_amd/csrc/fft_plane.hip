@@ -36,9 +36,14 @@ __global__ void __launch_bounds__(LY * LX / 16) k_fft_plane(const PlaneArgs a) {
   using GY = typename PG::GY;
   __shared__ cpx<float> lds[PG::LDS];
   const unsigned long long plane = blockIdx.x;
-  const long long base = (long long)plane * a.plane_stride;
+  long long base = (long long)plane * a.plane_stride, obase = base;
+  if (a.tab) {  // planes of a batch of transforms placed anywhere
+    const unsigned t = (unsigned)(plane / a.ppt), z = (unsigned)(plane % a.ppt);
+    base = (long long)a.tab[2 * t] + (long long)z * (LY * LX);
+    obase = (long long)a.tab[2 * t + 1] + (long long)z * (LY * LX);
+  }
   const cpx<float>* __restrict__ in = reinterpret_cast<const cpx<float>*>(a.in) + base;
-  cpx<float>* __restrict__ out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  cpx<float>* __restrict__ out = reinterpret_cast<cpx<float>*>(a.out) + obase;
   const int tid = threadIdx.x;
   cpx<float> v[16];
 
@@ -89,7 +94,7 @@ __global__ void __launch_bounds__(LY * LX / 16) k_fft_plane(const PlaneArgs a) {
         const int e = (j + r * NSL) * LX + c;
         cpx<float> x = v[b * RL + r];
         if (INV) x = cswap(x);
-        if (a.w_store) x = cscale(x, a.w_store[(unsigned long long)(base + e) & a.w_mask]);
+        if (a.w_store) x = cscale(x, a.w_store[(unsigned long long)(obase + e) & a.w_mask]);
         out[e] = x;
       }
     }
@@ -121,7 +126,9 @@ static bool dispatch_lx(int lx, const PlaneArgs& a, hipStream_t s) {
 
 int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (!plane_shape_supported(ly, lx) || a.nplanes == 0 || a.nplanes > 0xFFFFFFFFull / 1024) return -1;
+  if (!plane_shape_supported(ly, lx) || a.nplanes == 0 || a.nplanes > 0xFFFFFFFFull / 1024 ||
+      (a.tab && a.ppt == 0))
+    return -1;
   bool ok = false;
   switch (ly) {
     case 16: ok = dispatch_lx<16>(lx, a, s); break;

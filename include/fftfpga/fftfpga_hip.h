@@ -68,8 +68,9 @@ int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2 *d_in, float2 *d_out, bool 
  * transform in a single persistent kernel: per phase (axis), 256-thread
  * workgroups take chunks of tiles from a device-side work queue, largest
  * transforms first, with a grid barrier between phases.  Schedules (single
- * launch or one launch per phase, workgroups per CU, tiles per queue ticket)
- * are picked by fftamd_multi_tune and cached per shape signature. */
+ * launch, one launch per phase, or plane kernels per shape group then the z
+ * phase; workgroups per CU, tiles per queue ticket) are picked by
+ * fftamd_multi_tune and cached per shape signature. */
 typedef struct fftamd_multi_s *fftamd_multi_t;
 typedef struct fftamd_graph_s *fftamd_graph_t;
 
@@ -93,7 +94,12 @@ int fftamd_multi_check(fftamd_multi_t plan, void *stream);
  * after tuning, 1 when the schedule came from the cache, -1 on error. */
 int fftamd_multi_tune(fftamd_multi_t plan, void *stream, const char *cache_path, int reps,
                       double *best_us);
-/* mode 0: one launch with grid barriers; mode 1: one launch per phase. */
+/* mode 0: one launch with grid barriers; mode 1: one launch per phase;
+ * mode 2 (when every transform has three axes of >= 16 points with x-y planes
+ * of at most 128 x 128, single precision without split): the x and y passes
+ * of each shape group as one plane-kernel launch (a whole plane per
+ * workgroup, in LDS), then the z passes as one persistent launch.  The tuner
+ * considers every available mode. */
 int fftamd_multi_schedule(fftamd_multi_t plan, int *mode, unsigned *grid, unsigned *chunk,
                           unsigned *items, unsigned *phases);
 int fftamd_multi_set_schedule(fftamd_multi_t plan, int mode, unsigned grid, unsigned chunk);

@@ -1,9 +1,10 @@
 """Transforms of uneven sizes in one persistent launch (fftamd_multi_*).
 
 Bars: every transform of a mixed batch within rel-L2 2e-6 of the FP64
-oracle; the multi-transform kernel runs the same tile code as the
-per-transform path, so on the 448-transform workload its output must equal
-the per-transform output bit for bit; HIP-graph replays must equal the eager
+oracle; each schedule runs the same tile code as one per-transform path
+(modes 0 and 1: the line passes; mode 2: the plane kernel and the z line
+pass), so on the 448-transform workload its output must equal that path's
+output bit for bit; HIP-graph replays must equal the eager
 result on every replay; a grid-barrier wait forced past its bound from the
 host must be reported, not hang.
 """
@@ -79,7 +80,8 @@ def _config3_shapes():
     return [(n, n, n) for n in sizes]
 
 
-def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc):
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode):
     import torch
     from fftfpga_amd import c2c_dev
     from fftfpga_amd.multi import MultiPlan
@@ -88,12 +90,14 @@ def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc):
     host, d_in, offs = _pack(shapes, 2)
     d_out = torch.empty_like(d_in)
     plan = MultiPlan(shapes, d_in, d_out)
+    sch = plan.schedule()
+    plan.set_schedule(mode, sch["grid"], sch["chunk"])
     plan()
     plan.check()
     ref = torch.empty_like(d_in)
     for s, o in zip(shapes, offs):
         n = int(np.prod(s))
-        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s))
+        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s), plane_fusion=(mode == 2))
     torch.cuda.synchronize()
     assert torch.equal(d_out, ref)
     out = d_out.cpu().numpy()
@@ -163,7 +167,7 @@ def test_autotuner_picks_and_caches(gpu_lib, tmp_path):
     cache = str(tmp_path / "tune.txt")
     plan = MultiPlan(shapes, d_in, d_out)
     r1 = plan.tune(cache, reps=2)
-    assert not r1["from_cache"] and r1["best_us"] > 0 and r1["mode"] in (0, 1)
+    assert not r1["from_cache"] and r1["best_us"] > 0 and r1["mode"] in (0, 1, 2)
     plan2 = MultiPlan(shapes, d_in, d_out)
     r2 = plan2.tune(cache, reps=2)
     assert r2["from_cache"] and (r2["mode"], r2["grid"], r2["chunk"]) == (r1["mode"], r1["grid"], r1["chunk"])
