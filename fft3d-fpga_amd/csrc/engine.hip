@@ -261,9 +261,12 @@ static int split_strided(int prec, int lg, int* lg1, int* lg2) {
   return 0;
 }
 
-// Fewest planes for which the one-workgroup-per-plane pass (planes up to
-// 128 x 128) replaces the separate x and y passes.
+// Fewest planes for which a plane pass replaces the separate x and y passes:
+// any count for planes up to 128 x 128 (one workgroup per plane, measured
+// faster from one 64 x 64 plane up); 64 for 256 x 256 planes (four workgroups
+// per plane; fewer leave CUs idle and measured slower than the line passes).
 static constexpr unsigned long long PLANE_MIN_COUNT = 1;
+static constexpr unsigned long long PLANE256_MIN_COUNT = 64;
 
 static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec,
                       bool aliased, Plan& plan) {
@@ -290,7 +293,7 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
     unsigned long long O = batch;
     for (int d = 0; d < rank - 2; ++d) O *= dims[d];
     const bool big = LY == 256 && LX == 256;
-    if (big || (plane_shape_supported(LY, LX) && O >= PLANE_MIN_COUNT)) {
+    if ((big && O >= PLANE256_MIN_COUNT) || (plane_shape_supported(LY, LX) && O >= PLANE_MIN_COUNT)) {
       PassDesc p;
       p.lg = ilog2c(LX);
       p.mode = MODE_PLANE;
@@ -1419,9 +1422,7 @@ int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din
     return -1;
   }
   Plan* plan = nullptr;
-  // the fused plane pass reads its input in 16-byte pieces
-  const unsigned fl = ((uintptr_t)din & 15u) ? FLAG_NO_PLANE_FUSION : 0u;
-  if (get_plan(rank, dims, batch, fl, PREC_F32, din == dout, &plan)) return -1;
+  if (get_plan(rank, dims, batch, 0u, PREC_F32, din == dout, &plan)) return -1;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
   return run_plan(*plan, din, dout, inverse, s, w_load, w_store);
 }
@@ -1438,7 +1439,6 @@ int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* d
   }
   Plan* plan = nullptr;
   const bool aliased = din == dout;
-  if ((uintptr_t)din & 15u) flags |= FLAG_NO_PLANE_FUSION;  // the plane pass reads 16-byte pieces
   if (get_plan(rank, dims, batch, flags, prec, aliased, &plan)) return -1;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
   return run_plan(*plan, din, dout, inverse, s);

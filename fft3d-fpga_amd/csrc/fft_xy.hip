@@ -1,21 +1,21 @@
 // fft_xy.hip -- fused x+y pass over 256 x 256 planes: the 2D transform of
 // every plane of an array in one pass over HBM.
 //
-// A plane is 512 KiB, more than one CU's LDS, so four workgroups share it.
-// Row transform: 256 = 16 x 16, n = n1 + 16 n2, k = k1 + 16 k2.  Workgroup q
-// owns the output columns with k1 in [4q, 4q+4) (k = 16 k2 + 4q + u): it reads
-// the whole plane, evaluates only the four k1 outputs of each first-stage
-// 16-point DFT and only its four second-stage DFTs per row, so no arithmetic
-// is repeated across the four workgroups.  Its 64 columns stay in a 128 KiB
-// LDS buffer; then it runs the 256-point column transforms (y) on them and
-// writes them to the output plane.  The four workgroups of a plane are mapped
-// to one XCD and dispatched back to back (workgroups go to XCDs round-robin by
-// index), so the plane comes from HBM once and from that XCD's L2 for the
-// other three, and the four 32-byte pieces of each 128-byte output line meet
-// in the same L2.  A 3D transform then takes two passes over HBM (this one and
-// the z pass) instead of three.
+// A plane is 512 KiB, more than one CU's LDS, so four workgroups share it,
+// each owning 64 of the 256 output rows.  Column transform first: 256 =
+// 16 x 16, y = n1 + 16 n2, ky = k1 + 16 k2, and workgroup q owns the rows with
+// k1 in [4q, 4q+4) (ky = 16 k2 + 4q + u).  It reads the whole plane in
+// 512-byte row segments, evaluates only the four k1 outputs of each
+// first-stage 16-point DFT and only its four second-stage DFTs per column, so
+// no arithmetic is repeated across the four workgroups.  Its 64 rows stay in
+// LDS, where it runs their 256-point row transforms (x) and writes them as
+// whole output rows.  The four workgroups of a plane are mapped to one XCD and
+// dispatched back to back (workgroups go to XCDs round-robin by index), so the
+// plane comes from HBM once and from that XCD's L2 for the other three.  A 3D
+// transform then takes two passes over HBM (this one and the z pass) instead
+// of three.
 //
-// The pass reads the whole input plane and writes only its columns of the
+// The pass reads the whole input plane and writes only its rows of the
 // output, so it runs out of place.
 #include <hip/hip_runtime.h>
 
@@ -25,30 +25,21 @@
 namespace fftamd {
 
 constexpr int XY_L = 256;
-constexpr int XY_Q = 4;                 // workgroups per plane
-constexpr int XY_QC = XY_L / XY_Q;      // columns per workgroup
-constexpr int XY_NT = 1024;             // threads per workgroup
-constexpr int XY_TPR = 8;               // first row stage: threads per row
-constexpr int XY_ROWS = XY_NT / XY_TPR; // rows per first-stage chunk
+constexpr int XY_Q = 4;             // workgroups per plane
+constexpr int XY_QR = XY_L / XY_Q;  // output rows per workgroup
+constexpr int XY_NT = 1024;         // threads per workgroup
+using RowGeom = Geom<XY_L, MODE_CONTIG, XY_QR>;  // the workgroup's 64 rows in LDS
+static_assert(RowGeom::NT == XY_NT, "16 threads per row, 64 rows");
 
-// The 128 KiB LDS buffer holds, in turn:
-//  (1) the first-stage results, row y: 4 blocks (u) of 16 points (n1), read
-//      and written in 16-byte granules (two n1); granule G = 8u + n1/2 is
-//      XOR-swizzled inside its block so that the second stage's reads (16
-//      lanes = 4 rows x 4 blocks per step) hit distinct banks;
-//  (2) the row-transformed columns, [256 rows][64 columns], column index
-//      XOR-swizzled for the second stage's stores (4 rows x 4 columns per 16
-//      lanes); the column stage reads and writes whole rows.
-// Both maps are bijections within a row.
-__device__ __forceinline__ int xg_idx(int row, int G) {
-  return row * 32 + (G ^ (2 * (row & 3) + ((G >> 4) & 1)));
-}
-__device__ __forceinline__ int cb_idx(int row, int col) {
-  return row * XY_QC + (col ^ ((row & 7) << 2) ^ ((col >> 5) << 3));
-}
+// LDS buffer, in turn:
+//  (1) first-stage column results Y[n1][u][x] at (4 n1 + u) * 256 + x (16 x 4
+//      x 256 points; a wave reads or writes 64 consecutive x);
+//  (2) the 64 rows, RowGeom layout [row][x + x/16] (row r = 4 k2 + u holds
+//      output row ky = 16 k2 + 4q + u).
+constexpr int XY_LDS = RowGeom::LDS_ELEMS > 16 * 4 * XY_L ? RowGeom::LDS_ELEMS : 16 * 4 * XY_L;
 
-// output column of local column c of workgroup q (c = 4 k2 + u -> 16 k2 + 4 q + u)
-__device__ __forceinline__ int xy_gcol(int c, int q) { return 16 * (c >> 2) + 4 * q + (c & 3); }
+// output row of local row r of workgroup q
+__device__ __forceinline__ int xy_grow(int r, int q) { return 16 * (r >> 2) + 4 * q + (r & 3); }
 
 // x * (-i)^M
 template <int M>
@@ -94,78 +85,48 @@ __device__ __forceinline__ void twiddle256(cpx<float>* v, int k, const cpx<float
   v[15] = cmul(v[15], cmul(w12, w3));
 }
 
-// First-stage input of lane t for row y: complex 2t and 2t + 1 of the
-// 16-point groups n2 = N0 .. N0 + N - 1, in 16-byte loads.
-template <int N0, int N>
-__device__ __forceinline__ void load_rows(const cpx<float>* __restrict__ in, int y, int t, float4* p) {
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-    p[i] = *reinterpret_cast<const float4*>(in + (long long)y * XY_L + 16 * (N0 + i) + 2 * t);
-}
-
-// First stage for one chunk of rows: the 16-point DFTs over n2 of n1 = 2t
-// and 2t + 1, four outputs k1 = 4Q + u each, into the LDS granules.
-template <int Q, bool INV>
-__device__ __forceinline__ void first_stage(const PlaneArgs& a, const float4* p, unsigned long long plane, int y,
-                                            int t, float4* gbuf) {
-  cpx<float> x0[16], x1[16];
-#pragma unroll
-  for (int n2 = 0; n2 < 16; ++n2) {
-    cpx<float> v0 = {p[n2].x, p[n2].y}, v1 = {p[n2].z, p[n2].w};
-    if (a.w_load) {
-      const unsigned long long g = plane * a.plane_stride + (unsigned long long)(y * XY_L + 16 * n2 + 2 * t);
-      v0 = cscale(v0, a.w_load[g & a.w_mask]);
-      v1 = cscale(v1, a.w_load[(g + 1) & a.w_mask]);
-    }
-    x0[n2] = INV ? cswap(v0) : v0;
-    x1[n2] = INV ? cswap(v1) : v1;
-  }
-  cpx<float> o0[4], o1[4];
-  dft16_quarter<Q>(x0, o0);  // Y[2t][4Q + u]
-  dft16_quarter<Q>(x1, o1);  // Y[2t + 1][4Q + u]
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-    gbuf[xg_idx(y, 8 * u + t)] = make_float4(o0[u].x, o0[u].y, o1[u].x, o1[u].y);
-}
-
-// Row transforms of the whole plane for quarter Q; leaves the quarter's 64
-// columns in buf (column layout).  INV: inverse transform as
+// Column transforms of the whole plane for row quarter Q; leaves the
+// quarter's 64 rows in buf (RowGeom layout).  INV: inverse transform as
 // swap(DFT(swap(x))), the swaps being register renames.
 template <int Q, bool INV>
-__device__ __forceinline__ void row_stages(const PlaneArgs& a, const cpx<float>* __restrict__ in,
-                                           unsigned long long plane, cpx<float>* buf) {
+__device__ __forceinline__ void column_stages(const PlaneArgs& a, const cpx<float>* __restrict__ in,
+                                              unsigned long long plane, cpx<float>* buf) {
   const int tid = threadIdx.x;
-  float4* gbuf = reinterpret_cast<float4*>(buf);
-  // ---- first stage: 8 lanes per row, 128 rows per chunk
-  {
-    const int t = tid & (XY_TPR - 1), rl = tid / XY_TPR;
-#pragma unroll 1
-    for (int ch = 0; ch < XY_L / XY_ROWS; ++ch) {
-      float4 p[16];
-      load_rows<0, 16>(in, ch * XY_ROWS + rl, t, p);
-      first_stage<Q, INV>(a, p, plane, ch * XY_ROWS + rl, t, gbuf);
+  const int x = tid & (XY_L - 1), g = tid >> 8;  // column, group of four n1 (uniform per wave)
+  // ---- first stage: for n1 = 4g .. 4g+3, the 16-point DFT over n2 of
+  // column x at rows n1 + 16 n2, four outputs k1 = 4Q + u
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int n1 = 4 * g + m;
+    cpx<float> v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      const int e = (n1 + 16 * n2) * XY_L + x;
+      cpx<float> s = in[e];
+      if (a.w_load) s = cscale(s, a.w_load[(unsigned long long)(plane * a.plane_stride + e) & a.w_mask]);
+      v[n2] = INV ? cswap(s) : s;
     }
+    cpx<float> o[4];
+    dft16_quarter<Q>(v, o);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) buf[(4 * n1 + u) * XY_L + x] = o[u];
   }
   __syncthreads();
-  // ---- second stage: lane (y, u) runs X[k1 + 16 k2] = DFT16_n1(Y[n1][k1] W256^{n1 k1}), k1 = 4Q + u
-  const int y = tid >> 2, u = tid & 3;
+  // ---- second stage: lane (x, u = g): X[k1 + 16 k2] = DFT16_n1(Y[n1][k1] W256^{n1 k1}), k1 = 4Q + u
+  const int u = g;
   cpx<float> v[16];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const float4 p = gbuf[xg_idx(y, 8 * u + i)];
-    v[2 * i] = {p.x, p.y};
-    v[2 * i + 1] = {p.z, p.w};
-  }
-  twiddle256(v, 4 * Q + u, reinterpret_cast<const cpx<float>*>(a.tw_x));
+  for (int n1 = 0; n1 < 16; ++n1) v[n1] = buf[(4 * n1 + u) * XY_L + x];
+  twiddle256(v, 4 * Q + u, reinterpret_cast<const cpx<float>*>(a.tw_y));
   dft16(v);
-  __syncthreads();  // every first-stage granule has been read
+  __syncthreads();  // every first-stage value has been read
 #pragma unroll
-  for (int k2 = 0; k2 < 16; ++k2) buf[cb_idx(y, 4 * k2 + u)] = v[k2];
+  for (int k2 = 0; k2 < 16; ++k2) buf[RowGeom::idx(4 * k2 + u, x)] = v[k2];
 }
 
 template <bool INV>
 __global__ void __launch_bounds__(XY_NT) k_fft_xy256(const PlaneArgs a) {
-  __shared__ __align__(16) cpx<float> buf[XY_L * XY_QC];
+  __shared__ cpx<float> buf[XY_LDS];
   // XCD-aware mapping: block b runs on XCD b % 8 (dispatch is round-robin
   // over XCDs); the k-th block of an XCD takes quarter k % 4 of plane
   // (k / 4) * 8 + XCD, so the four quarters of a plane are consecutive blocks
@@ -177,37 +138,32 @@ __global__ void __launch_bounds__(XY_NT) k_fft_xy256(const PlaneArgs a) {
   if (plane >= a.nplanes) return;  // uniform over the workgroup
   const cpx<float>* __restrict__ in = reinterpret_cast<const cpx<float>*>(a.in) + plane * a.plane_stride;
   cpx<float>* __restrict__ out = reinterpret_cast<cpx<float>*>(a.out) + plane * a.plane_stride;
-  const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw_x);
-  const int tid = threadIdx.x;
 
   switch (q) {  // uniform: the quarter is a compile-time constant inside
-    case 0: row_stages<0, INV>(a, in, plane, buf); break;
-    case 1: row_stages<1, INV>(a, in, plane, buf); break;
-    case 2: row_stages<2, INV>(a, in, plane, buf); break;
-    default: row_stages<3, INV>(a, in, plane, buf); break;
+    case 0: column_stages<0, INV>(a, in, plane, buf); break;
+    case 1: column_stages<1, INV>(a, in, plane, buf); break;
+    case 2: column_stages<2, INV>(a, in, plane, buf); break;
+    default: column_stages<3, INV>(a, in, plane, buf); break;
   }
   __syncthreads();
 
-  // ---- y: 64 columns x 256 points; thread (column c, butterfly tau)
-  const int c = tid & (XY_QC - 1), tau = tid >> 6;
+  // ---- rows: thread (local row c, tau), the line passes' stages on LDS rows
+  const int tid = threadIdx.x;
+  const int c = tid / RowGeom::TPL, tau = tid % RowGeom::TPL;
   cpx<float> v[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = buf[cb_idx(tau + 16 * r, c)];
-  dft16(v);
-  __syncthreads();  // every thread read its column inputs
-#pragma unroll
-  for (int r = 0; r < 16; ++r) buf[cb_idx(16 * tau + r, c)] = v[r];
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = buf[cb_idx(tau + 16 * r, c)];
-  twiddle256(v, tau, tw);
-  dft16(v);
+  stage_load_lds<float, XY_L, MODE_CONTIG, 0, XY_QR>(v, buf, c, tau);
+  __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
+  run_stages<float, XY_L, MODE_CONTIG, 0, false, XY_QR>(v, buf, c, tau,
+                                                        reinterpret_cast<const cpx<float>*>(a.tw_x));
+  // last stage: point kx = tau + 16 r of the row; 16 lanes write 128 contiguous bytes
+  static_assert(sched_np(XY_L) == 2 && RowGeom::TPL == 16, "256 = 16 x 16, one butterfly per lane");
+  const long long row = (long long)xy_grow(c, (int)q) * XY_L;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    cpx<float> x = INV ? cswap(v[r]) : v[r];
-    const long long e = (long long)(tau + 16 * r) * XY_L + xy_gcol(c, (int)q);
-    if (a.w_store) x = cscale(x, a.w_store[(unsigned long long)(plane * a.plane_stride + e) & a.w_mask]);
-    out[e] = x;
+    cpx<float> s = INV ? cswap(v[r]) : v[r];
+    const long long e = row + tau + 16 * r;
+    if (a.w_store) s = cscale(s, a.w_store[(unsigned long long)(plane * a.plane_stride + e) & a.w_mask]);
+    out[e] = s;
   }
 }
 

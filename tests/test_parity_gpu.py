@@ -303,15 +303,15 @@ def test_device_api_orders_with_torch_default_stream(gpu_lib):
 
 # ---- fused 256 x 256 plane pass ------------------------------------------------------------------
 
-@pytest.mark.parametrize("shape,rank", [((1, 256, 256), 2), ((9, 256, 256), 2), ((4, 256, 256), 3),
-                                        ((2, 8, 256, 256), 3), ((1, 128, 128), 2), ((3, 64, 128), 2),
+@pytest.mark.parametrize("shape,rank", [((64, 256, 256), 2), ((65, 256, 256), 2), ((2, 64, 256, 256), 3),
+                                        ((4, 256, 256), 3), ((1, 128, 128), 2), ((3, 64, 128), 2),
                                         ((2, 128, 64), 3), ((16, 32, 32), 3), ((5, 16, 64), 2),
                                         ((128, 128, 128), 3), ((3, 64, 64, 64), 3)])
 def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
-    """256 x 256 planes: counts below, at and above one per XCD slot (1, 4, 9,
-    16); planes up to 128 x 128 (one workgroup each): square and rectangular
-    shapes.  Out of place and in place; against the FP64 oracle and the
-    unfused path."""
+    """256 x 256 planes: 64, 65 (a partial XCD slot) and 128 planes, and 4
+    (below the plane-pass threshold: line passes); planes up to 128 x 128
+    (one workgroup each): square and rectangular shapes.  Out of place and in
+    place; against the FP64 oracle and the unfused path."""
     import torch
     from fftfpga_amd import c2c_dev
     n = int(np.prod(shape))
@@ -334,7 +334,7 @@ def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
     assert np.array_equal(xt.cpu().numpy().reshape(-1), x)  # input untouched
 
 
-@pytest.mark.parametrize("shape,rank", [((2, 256, 256), 2), ((4, 256, 256), 3), ((2, 128, 128), 2),
+@pytest.mark.parametrize("shape,rank", [((64, 256, 256), 2), ((64, 256, 256), 3), ((2, 128, 128), 2),
                                         ((4, 64, 64), 3)])
 def test_plane_pass_weights(gpu_lib, orc, shape, rank):
     """Weights applied in the fused pass's loads (both ranks) and stores (rank 2)."""
@@ -360,15 +360,15 @@ def test_plane_pass_weights(gpu_lib, orc, shape, rank):
 
 
 def test_plane_pass_unaligned_input(gpu_lib, orc):
-    """An input that is only 8-byte aligned (a view at an odd element offset)
-    takes the separate x and y passes."""
+    """An input that is only 8-byte aligned (a view at an odd element offset):
+    the plane pass reads single complex elements, so it takes it as is."""
     import torch
     from fftfpga_amd import c2c_dev
-    x = orc.uniform(2 * 65536, 44)
-    base = torch.zeros(2 * 65536 + 1, dtype=torch.complex64, device="cuda")
+    x = orc.uniform(64 * 65536, 44)
+    base = torch.zeros(64 * 65536 + 1, dtype=torch.complex64, device="cuda")
     base[1:] = torch.from_numpy(x).cuda()
-    xt = base[1:].view(2, 256, 256)
+    xt = base[1:].view(64, 256, 256)
     assert xt.data_ptr() % 16 == 8
     y = c2c_dev(xt, 2)
     torch.cuda.synchronize()
-    assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [256, 256], batch=2)) <= TOL
+    assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [256, 256], batch=64)) <= TOL

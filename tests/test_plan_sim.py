@@ -171,6 +171,7 @@ CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16]
          ([2, 1, 4], 1, 0), ([256], 5, 1), ([1 << 14], 2, 1), ([1 << 14], 2, 0), ([1 << 16], 1, 1),
          ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0), ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
          ([256, 256], 1, 4), ([2, 256, 256], 1, 4), ([4, 128, 128], 1, 0), ([128, 64], 3, 0),
+         ([64, 256, 256], 1, 0), ([256, 256], 65, 0),
          ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4)]
 
 
@@ -190,7 +191,8 @@ def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
 
 @pytest.mark.parametrize("dims,batch,flags", [([64, 64, 64], 1, 0), ([2048, 32], 1, 0),
                                               ([1 << 14], 1, 1), ([1 << 14], 1, 0),
-                                              ([2, 256, 256], 1, 0), ([256, 256], 2, 0)])
+                                              ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
+                                              ([64, 256, 256], 1, 0)])
 def test_plan_replay_in_place(fl, orc, dims, batch, flags):
     passes, final_copy = describe(fl, dims, batch, flags, aliased=True)
     n = int(np.prod(dims)) * batch
@@ -244,16 +246,13 @@ def test_bitrev_epilogue_runs(lg):
 
 # ---- fused 256 x 256 plane pass ------------------------------------------------
 
-def _cb_idx(row, col):
-    return row * 64 + (col ^ ((row & 7) << 2) ^ ((col >> 5) << 3))
+def _row_idx(r, p):
+    """LDS index of point p of local row r (the line passes' [row][p + p/16] layout)."""
+    return r * 272 + p + (p >> 4)
 
 
-def _xg_idx(row, G):
-    return row * 32 + (G ^ (2 * (row & 3) + ((G >> 4) & 1)))
-
-
-def _gcol(c, q):
-    return 16 * (c >> 2) + 4 * q + (c & 3)
+def _grow(r, q):
+    return 16 * (r >> 2) + 4 * q + (r & 3)
 
 
 def _dft16_quarter(v, g):
@@ -286,44 +285,35 @@ def emulate_plane256(plane):
     w256 = lambda j: np.exp(-2j * np.pi * (j % 256) / 256)  # noqa: E731
     r16 = np.arange(16)
     for q in range(4):
-        # first row stage into 16-byte granules (pairs of complex)
-        gbuf = np.full((256 * 32, 2), np.nan + 0j)
-        t, rl = tid & 7, tid >> 3
-        for ch in range(2):
-            y = ch * 128 + rl
-            x0 = plane[y[:, None], 16 * r16[None, :] + 2 * t[:, None]]
-            x1 = plane[y[:, None], 16 * r16[None, :] + 2 * t[:, None] + 1]
-            o0, o1 = _dft16_quarter(x0, q), _dft16_quarter(x1, q)
+        # column first stage: lane (x, g), n1 = 4g + m, outputs k1 = 4q + u
+        x, g = tid & 255, tid >> 8
+        ybuf = np.full(16 * 4 * 256, np.nan + 0j)
+        for m in range(4):
+            n1 = 4 * g + m
+            v = plane[n1[:, None] + 16 * r16[None, :], x[:, None]]
+            o = _dft16_quarter(v, q)
             for u in range(4):
-                g = _xg_idx(y, 8 * u + t)
-                assert np.isnan(gbuf[g]).all()
-                gbuf[g, 0], gbuf[g, 1] = o0[:, u], o1[:, u]
-        assert not np.isnan(gbuf).any()
-        # second row stage: lane (y, u), k1 = 4q + u
-        y, u = tid >> 2, tid & 3
-        v = np.zeros((1024, 16), complex)
-        for i in range(8):
-            g = _xg_idx(y, 8 * u + i)
-            v[:, 2 * i], v[:, 2 * i + 1] = gbuf[g, 0], gbuf[g, 1]
+                idx = (4 * n1 + u) * 256 + x
+                assert np.isnan(ybuf[idx]).all()
+                ybuf[idx] = o[:, u]
+        assert not np.isnan(ybuf).any()
+        # column second stage: lane (x, u = g), k1 = 4q + u
+        u = g
+        v = ybuf[(4 * r16[None, :] + u[:, None]) * 256 + x[:, None]]
         v = np.fft.fft(v * w256((4 * q + u)[:, None] * r16[None, :]), axis=1)
-        colbuf = np.full(256 * 64, np.nan + 0j)
+        rows = np.full(64 * 272, np.nan + 0j)
         for k2 in range(16):
-            idx = _cb_idx(y, 4 * k2 + u)
-            assert np.isnan(colbuf[idx]).all()
-            colbuf[idx] = v[:, k2]
-        assert not np.isnan(colbuf).any()
-        # column stage
-        c, tau = tid & 63, tid >> 6
-        v = colbuf[_cb_idx(tau[:, None] + 16 * r16[None, :], c[:, None])]
-        v = np.fft.fft(v, axis=1)
-        new = np.full(256 * 64, np.nan + 0j)
-        new[_cb_idx(16 * tau[:, None] + r16[None, :], c[:, None])] = v
-        assert not np.isnan(new).any()
-        v = new[_cb_idx(tau[:, None] + 16 * r16[None, :], c[:, None])]
-        v = np.fft.fft(v * w256(tau[:, None] * r16[None, :]), axis=1)
-        rows, cols = tau[:, None] + 16 * r16[None, :], _gcol(c, q)[:, None]
-        assert np.isnan(out[rows, cols]).all()
-        out[rows, cols] = v
+            idx = _row_idx(4 * k2 + u, x)
+            assert np.isnan(rows[idx]).all()
+            rows[idx] = v[:, k2]
+        # row transforms of the 64 local rows (the line-pass stages, checked
+        # separately), written to output row 16 k2 + 4q + u
+        pts = np.arange(256)
+        for r in range(64):
+            line = rows[_row_idx(r, pts)]
+            assert not np.isnan(line).any()
+            assert np.isnan(out[_grow(r, q)]).all()
+            out[_grow(r, q)] = np.fft.fft(line)
     return out
 
 
@@ -333,13 +323,11 @@ def test_plane256_index_maps():
     got = emulate_plane256(plane)
     assert not np.isnan(got).any()
     assert np.allclose(got, np.fft.fft2(plane), rtol=1e-12, atol=1e-9)
-    # both swizzled LDS maps are bijections onto the buffer
-    rows, cols = np.meshgrid(np.arange(256), np.arange(64), indexing="ij")
-    idx = _cb_idx(rows, cols)
-    assert idx.min() == 0 and idx.max() == 256 * 64 - 1 and np.unique(idx).size == idx.size
-    rows, gr = np.meshgrid(np.arange(256), np.arange(32), indexing="ij")
-    idx = _xg_idx(rows, gr)
-    assert idx.min() == 0 and idx.max() == 256 * 32 - 1 and np.unique(idx).size == idx.size
+    # the 64 LDS rows fit the buffer and do not overlap
+    r, p = np.meshgrid(np.arange(64), np.arange(256), indexing="ij")
+    idx = _row_idx(r, p)
+    assert idx.max() < 64 * 272 and np.unique(idx).size == idx.size
+    assert sorted(_grow(r, q) for q in range(4) for r in range(64)) == list(range(256))
 
 
 @pytest.mark.parametrize("nplanes", [1, 7, 8, 9, 256, 1000])
