@@ -232,36 +232,79 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
   constexpr int NBL = PPT / RL;
   if constexpr (EPI == EPI_BITREV) {
     // Point k = j + r*NSL (r < RL) lands at bitrev(k) = bitrev(r) + RL*bitrev(j):
-    // each butterfly's RL outputs form one contiguous run of the output line,
-    // so the permuted line is written straight from registers in RL*8-byte runs.
+    // each butterfly's RL outputs form one contiguous run of the output line.
     constexpr unsigned LGR = ilog2c(RL), LGJ = ilog2c(L) - ilog2c(RL);
+    if constexpr (NP > 1) {
+      // through LDS: the runs are placed at their permuted positions, then the
+      // line is written in order, each store instruction one contiguous span
+      // (stores straight from registers would give every lane its own line)
+      __syncthreads();  // the last stage has read its inputs
 #pragma unroll
-    for (int b = 0; b < NBL; ++b) {
-      const int j = tau + b * TPL;
-      const long long run = (long long)RL * brev_bits((unsigned)j, LGJ);
+      for (int b = 0; b < NBL; ++b) {
+        const int j = tau + b * TPL;
+        const int run = RL * (int)brev_bits((unsigned)j, LGJ);
 #pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const long long q = run + brev_bits((unsigned)r, LGR);
-        cpx<R> x = v[b * RL + r];
+        for (int r = 0; r < RL; ++r) lds[G::idx(c, run + (int)brev_bits((unsigned)r, LGR))] = v[b * RL + r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < PPT; ++i) {
+        const long long q = tau + i * TPL;
+        cpx<R> x = lds[G::idx(c, (int)q)];
         if (inv) x = cswap(x);
         if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + q * a.out_p) & a.w_mask]);
         if (active) out[q * a.out_p] = x;
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < NBL; ++b) {
+        const int j = tau + b * TPL;
+        const long long run = (long long)RL * brev_bits((unsigned)j, LGJ);
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const long long q = run + brev_bits((unsigned)r, LGR);
+          cpx<R> x = v[b * RL + r];
+          if (inv) x = cswap(x);
+          if (a.w_store) x = cscale(x, (R)a.w_store[(unsigned long long)(out_off + q * a.out_p) & a.w_mask]);
+          if (active) out[q * a.out_p] = x;
+        }
       }
     }
   } else {
 #pragma unroll
     for (int b = 0; b < NBL; ++b) {
       const int j = tau + b * TPL;
+      // four-step post-twiddle W_M^{cc k}, cc = sub-axis index of this column,
+      // k = j + r*NSL: W^{cc j} times the table powers W^{cc NSL m}, m = 1, 2,
+      // 4, 8 (each from the two-level table: two loads and one product)
+      cpx<R> base{}, s1{}, s2{}, s4{}, s8{};
+      if constexpr (EPI == EPI_TWIDDLE) {
+        const unsigned cc = col >> a.ptw_cshift;
+        const cpx<R>* lo = reinterpret_cast<const cpx<R>*>(a.ptw_lo);
+        const cpx<R>* hi = reinterpret_cast<const cpx<R>*>(a.ptw_hi);
+        const unsigned lomask = (1u << a.ptw_hb) - 1u;
+        auto w2 = [&](unsigned t) {
+          t &= a.ptw_mmask;
+          return cmul(hi[t >> a.ptw_hb], lo[t & lomask]);
+        };
+        base = w2(cc * (unsigned)j);
+        const unsigned st = cc * (unsigned)NSL;
+        if constexpr (RL > 1) s1 = w2(st);
+        if constexpr (RL > 2) s2 = w2(2u * st);
+        if constexpr (RL > 4) s4 = w2(4u * st);
+        if constexpr (RL > 8) s8 = w2(8u * st);
+      }
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
         const int k = j + r * NSL;
         cpx<R> x = v[b * RL + r];
         if constexpr (EPI == EPI_TWIDDLE) {
-          // W_M^t, t = (sub-axis index of this column) * k mod M, from a two-level table
-          const unsigned t = ((col >> a.ptw_cshift) * (unsigned)k) & a.ptw_mmask;
-          const cpx<R> wl = reinterpret_cast<const cpx<R>*>(a.ptw_lo)[t & ((1u << a.ptw_hb) - 1u)];
-          const cpx<R> wh = reinterpret_cast<const cpx<R>*>(a.ptw_hi)[t >> a.ptw_hb];
-          x = cmul(x, cmul(wh, wl));
+          cpx<R> w = base;
+          if (r & 1) w = cmul(w, s1);
+          if (r & 2) w = cmul(w, s2);
+          if (r & 4) w = cmul(w, s4);
+          if (r & 8) w = cmul(w, s8);
+          x = cmul(x, w);
         }
         if (inv) x = cswap(x);
         if (active) {
