@@ -13,6 +13,12 @@ roofline = the dominant kernel: algorithmic bytes of one launch (16 B per point:
          fraction for the whole transform (16 B per point per transform).
 cpu_baseline = the FP64 oracle (a scalar C port, 1 thread) on one forward
          transform of the same workload, timed on this host.
+Further objects on the same line (default run): unfused_passes (the same
+steps with 256 x 256 planes as separate x and y passes), torch_fft_rocfft
+(torch.fft on the same data), batched (448 uneven transforms: persistent /
+grouped-plane launch against a per-transform loop), split_512 (split-precision
+512^3), autograd_16x128 (differentiable forward/backward), cpu_fftwf (threaded
+FFTW API on this host).
 """
 from __future__ import annotations
 
@@ -208,18 +214,19 @@ def autograd_16x128(steps):
         y.backward(g)
     torch.cuda.synchronize()
     tf = tb = 0.0
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     for _ in range(steps):
         x.grad = None
-        t0 = time.perf_counter()
+        torch.cuda.synchronize()
+        e0.record()
         y = fft3(x, w)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+        e1.record()
         y.backward(g)
+        e2.record()
         torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        tf += t1 - t0
-        tb += t2 - t1
-    ms_f, ms_b = tf * 1e3 / steps, tb * 1e3 / steps
+        tf += e0.elapsed_time(e1)
+        tb += e1.elapsed_time(e2)
+    ms_f, ms_b = tf / steps, tb / steps
     alg_f, alg_b = 16.0 * B * N ** 3, 16.0 * B * N ** 3 + 4.0 * N ** 3
     del x, w, g, y
     torch.cuda.empty_cache()
@@ -229,7 +236,7 @@ def autograd_16x128(steps):
             "hbm_gbs_backward": round(alg_b / (ms_b * 1e-3) / 1e9, 1),
             "roofline_frac_forward": round(alg_f / (ms_f * 1e-3) / HBM_PEAK, 4),
             "roofline_frac_backward": round(alg_b / (ms_b * 1e-3) / HBM_PEAK, 4),
-            "timing": "host clock around each call with a device synchronisation (includes launch overhead)"}
+            "timing": "events on the current stream around the forward call and around backward()"}
 
 
 BATCH_COUNTS = {32: 256, 64: 160, 128: 32}
