@@ -48,10 +48,12 @@ constexpr int geom_tpl(int L) { return L / geom_ppt(L); }
 // Lines per workgroup.  Contiguous lines: 256 threads per workgroup (one line
 // per workgroup from L = 4096 up).  Strided lines: at least 16 adjacent
 // columns (128-byte row segments; narrower segments measured far slower) and
-// 256 threads up to L = 256.
+// 256 threads up to L = 256; 32 columns (256-byte segments, 1024 threads) at
+// L = 512, where rows lie 4 KiB to 2 MiB apart (measured 7-18% faster than 16
+// on the y and z passes of a 512^3 array; slower at L = 256).
 constexpr int geom_cols(int L, int mode) {
   return mode == MODE_CONTIG ? (geom_tpl(L) >= 256 ? 1 : 256 / geom_tpl(L))
-                             : (L >= 256 ? 16 : 256 / geom_tpl(L));
+                             : (L >= 512 ? 32 : L >= 256 ? 16 : 256 / geom_tpl(L));
 }
 constexpr int geom_threads(int L, int mode) { return geom_cols(L, mode) * geom_tpl(L); }
 
