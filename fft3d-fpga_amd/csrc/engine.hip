@@ -611,7 +611,7 @@ static double ms_between(hipEvent_t a, hipEvent_t b);
 struct PlaneGroup {
   int ly = 0, lx = 0;
   unsigned ppt = 0, count = 0;  // planes per transform, transforms
-  size_t tab_off = 0;           // first (in, out) pair in the device table
+  size_t tab_off = 0;           // first (in, out) pair of the group in the device tables
 };
 
 struct MultiPlan {
@@ -625,12 +625,14 @@ struct MultiPlan {
   MultiCtl* d_ctl = nullptr;
   unsigned grid = 0, chunk = 4;
   // 0: one launch, grid barriers between phases; 1: one launch per phase;
-  // 2: x and y of each shape group as plane kernels, then the z phase
+  // 2: x and y of each shape group as one plane-kernel launch, then its z
+  // passes as one line-pass launch (transform tables; no persistent kernel)
   int mode = 0;
   bool split = false;
   bool plane_ok = false;  // mode 2 available (every transform 3D, planes up to 128 x 128)
   std::vector<PlaneGroup> groups;
-  unsigned long long* d_tab = nullptr;
+  unsigned long long* d_tab = nullptr;    // (input, output) offset per transform, by group
+  unsigned long long* d_tab_z = nullptr;  // (output, output): the z passes run in place
   const void* din = nullptr;
   void* dout = nullptr;
   bool inverse = false;
@@ -642,6 +644,7 @@ struct MultiPlan {
     if (d_phase) (void)hipFree(d_phase);
     if (d_ctl) (void)hipFree(d_ctl);
     if (d_tab) (void)hipFree(d_tab);
+    if (d_tab_z) (void)hipFree(d_tab_z);
   }
 };
 
@@ -789,7 +792,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
       const unsigned i = order[oi];
       by_shape[std::make_tuple(dims3[3 * i], dims3[3 * i + 1], dims3[3 * i + 2])].push_back(i);
     }
-    std::vector<unsigned long long> tab;
+    std::vector<unsigned long long> tab, tab_z;
     for (auto it = by_shape.rbegin(); it != by_shape.rend(); ++it) {  // largest shapes first
       PlaneGroup g;
       g.ppt = std::get<0>(it->first);
@@ -800,11 +803,16 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
       for (unsigned i : it->second) {
         tab.push_back(in_off[i]);
         tab.push_back(out_off[i]);
+        tab_z.push_back(out_off[i]);
+        tab_z.push_back(out_off[i]);
       }
       mp->groups.push_back(g);
     }
     HIP_TRY(hipMalloc(&mp->d_tab, sizeof(unsigned long long) * tab.size()));
     HIP_TRY(hipMemcpy(mp->d_tab, tab.data(), sizeof(unsigned long long) * tab.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&mp->d_tab_z, sizeof(unsigned long long) * tab_z.size()));
+    HIP_TRY(hipMemcpy(mp->d_tab_z, tab_z.data(), sizeof(unsigned long long) * tab_z.size(),
+                      hipMemcpyHostToDevice));
   }
   unsigned per = 0;
   if (multi_default_grid(&mp->grid, &per, mp->split)) return -1;
@@ -836,10 +844,25 @@ int multi_exec(void* handle, void* stream) {
         return -1;
       }
     }
-    if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, 2, 1, mp->d_ctl, mp->grid, mp->chunk,
-                     g_multi_spin_limit, stream, mp->split)) {
-      set_error("multi-transform launch failed: %s", hipGetErrorString(hipGetLastError()));
-      return -1;
+    // z axis of each group: one strided line pass over all its transforms, in place
+    for (const PlaneGroup& g : mp->groups) {
+      const long long L = g.ppt, s = (long long)g.ly * g.lx;
+      PassArgs z{};
+      z.in = mp->dout;
+      z.out = mp->dout;
+      z.in_p = z.out_p = s;
+      z.in_c = z.out_c = 1;
+      z.ncol = (unsigned)s;
+      z.ncb = (unsigned)((s + geom_cols((int)L, MODE_STRIDED) - 1) / geom_cols((int)L, MODE_STRIDED));
+      z.K = 1;
+      z.inverse = mp->inverse ? 1 : 0;
+      z.tab = mp->d_tab_z + 2 * g.tab_off;
+      if (sched_np((int)L) > 1 && get_stockham(PREC_F32, (int)L, &z.tw)) return -1;
+      if (launch_fft_pass(PREC_F32, ilog2c(L), MODE_STRIDED, EPI_NONE, z,
+                          (unsigned long long)z.ncb * g.count, stream, false)) {
+        set_error("z pass launch failed: %s", hipGetErrorString(hipGetLastError()));
+        return -1;
+      }
     }
     return 0;
   }
@@ -949,6 +972,8 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
   for (int mode = 0; mode <= (mp->plane_ok ? 2 : 1); ++mode)
     for (unsigned pc = 1; pc <= per; ++pc)
       for (unsigned ch : chunks) {
+        // mode 2 launches plain kernels: grid and chunk do not apply
+        if (mode == 2 && (pc != per || ch != chunks[0])) continue;
         mp->mode = mode;
         mp->grid = pc * ncu;
         mp->chunk = ch;

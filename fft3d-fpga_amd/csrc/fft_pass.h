@@ -30,6 +30,9 @@ struct PassArgs {
   const float* w_load;
   const float* w_store;
   unsigned long long w_mask;
+  // optional transform table: outer index o starts at tab[2 o] in `in` and
+  // tab[2 o + 1] in `out` (complex elements) instead of o * in_o, o * out_o
+  const unsigned long long* tab;
 };
 
 // Launches one pass; returns 0, -1 for an unsupported (prec, lg, mode, epi), -2 on a launch error.

@@ -164,8 +164,10 @@ __device__ __forceinline__ TileLoc tile_locate(const PassArgs& a, unsigned long 
   t.col = cb * G::C + t.c;
   t.active = t.col < a.ncol;
   const unsigned ko = a.kbits ? brev_bits(kk, a.kbits) : kk;
-  t.in_off = (long long)o * a.in_o + (long long)kk * a.in_k + (long long)t.col * a.in_c;
-  t.out_off = (long long)o * a.out_o + (long long)ko * a.out_k + (long long)t.col * a.out_c;
+  const long long ib = a.tab ? (long long)a.tab[2 * o] : (long long)o * a.in_o;
+  const long long ob = a.tab ? (long long)a.tab[2 * o + 1] : (long long)o * a.out_o;
+  t.in_off = ib + (long long)kk * a.in_k + (long long)t.col * a.in_c;
+  t.out_off = ob + (long long)ko * a.out_k + (long long)t.col * a.out_c;
   return t;
 }
 

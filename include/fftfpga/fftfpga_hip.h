@@ -68,8 +68,8 @@ int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2 *d_in, float2 *d_out, bool 
  * transform in a single persistent kernel: per phase (axis), 256-thread
  * workgroups take chunks of tiles from a device-side work queue, largest
  * transforms first, with a grid barrier between phases.  Schedules (single
- * launch, one launch per phase, or plane kernels per shape group then the z
- * phase; workgroups per CU, tiles per queue ticket) are picked by
+ * launch, one launch per phase, or plane and z-pass kernels per shape group;
+ * workgroups per CU, tiles per queue ticket) are picked by
  * fftamd_multi_tune and cached per shape signature. */
 typedef struct fftamd_multi_s *fftamd_multi_t;
 typedef struct fftamd_graph_s *fftamd_graph_t;
@@ -96,10 +96,11 @@ int fftamd_multi_tune(fftamd_multi_t plan, void *stream, const char *cache_path,
                       double *best_us);
 /* mode 0: one launch with grid barriers; mode 1: one launch per phase;
  * mode 2 (when every transform has three axes of >= 16 points with x-y planes
- * of at most 128 x 128, single precision without split): the x and y passes
- * of each shape group as one plane-kernel launch (a whole plane per
- * workgroup, in LDS), then the z passes as one persistent launch.  The tuner
- * considers every available mode. */
+ * of at most 128 x 128, single precision without split): per shape group, the
+ * x and y passes as one plane-kernel launch (a whole plane per workgroup, in
+ * LDS) and the z passes as one line-pass launch, both over a table of the
+ * group's transforms (grid and chunk do not apply).  The tuner considers
+ * every available mode. */
 int fftamd_multi_schedule(fftamd_multi_t plan, int *mode, unsigned *grid, unsigned *chunk,
                           unsigned *items, unsigned *phases);
 int fftamd_multi_set_schedule(fftamd_multi_t plan, int mode, unsigned grid, unsigned chunk);
