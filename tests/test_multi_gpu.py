@@ -74,6 +74,56 @@ def test_mixed_batch_in_place(gpu_lib, orc):
         assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s))) <= TOL
 
 
+PLANES = [(32, 32, 32)] * 5 + [(64, 64, 64)] * 3 + [(16, 128, 128)] * 2 + [(128, 64, 32), (64, 32, 128)]
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("in_place", [False, True])
+def test_plane_schedule_vs_oracle(gpu_lib, orc, inverse, in_place):
+    """Mode 2 (plane kernels per shape group, then table-driven z passes) on
+    transforms placed in reverse order with gaps between them."""
+    import torch
+    from fftfpga_amd import datagen
+    from fftfpga_amd.multi import MultiPlan
+    sizes = [int(np.prod(s)) for s in PLANES]
+    offs, o = [], 0
+    for n in reversed(sizes):  # last transform first, 13-element gaps
+        offs.append(o)
+        o += n + 13
+    offs = offs[::-1]
+    host = np.zeros(o, np.complex64)
+    for i, (n, off) in enumerate(zip(sizes, offs)):
+        host[off:off + n] = datagen.gauss(n, datagen.member_seed(8, i))
+    d_in = torch.from_numpy(host).cuda()
+    d_out = d_in if in_place else torch.zeros_like(d_in)
+    plan = MultiPlan(PLANES, d_in, d_out, in_offsets=offs, out_offsets=offs, inverse=inverse)
+    sch = plan.schedule()
+    plan.set_schedule(2, sch["grid"], sch["chunk"])
+    plan()
+    plan.check()
+    out = d_out.cpu().numpy()
+    for s, n, off in zip(PLANES, sizes, offs):
+        ref = orc.fftnd(host[off:off + n], list(s), inverse=inverse)
+        assert orc.rel_l2(out[off:off + n], ref) <= TOL, s
+    if not in_place:  # gaps untouched
+        gaps = np.ones(o, bool)
+        for n, off in zip(sizes, offs):
+            gaps[off:off + n] = False
+        assert not out[gaps].any()
+    plan.close()
+
+
+def test_plane_schedule_unavailable_is_rejected(gpu_lib):
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    _, d_in, _ = _pack(MIXED, 3)
+    plan = MultiPlan(MIXED, d_in, torch.empty_like(d_in))  # 8^3 planes: no plane kernel
+    sch = plan.schedule()
+    with pytest.raises(ValueError):
+        plan.set_schedule(2, sch["grid"], sch["chunk"])
+    plan.close()
+
+
 def _config3_shapes():
     from fftfpga_amd import datagen
     sizes = datagen.batch_order({32: 256, 64: 160, 128: 32}, 2)
