@@ -288,12 +288,13 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   // workgroups per plane, out of place) or up to 128 x 128 (one workgroup
   // per plane in LDS, in place allowed)
   int top = rank - 1;
-  if (rank >= 2 && prec == PREC_F32 && !plan.split && !(flags & FLAG_NO_PLANE_FUSION)) {
+  if (rank >= 2 && prec == PREC_F32 && !(flags & FLAG_NO_PLANE_FUSION)) {
     const long long LY = (long long)dims[rank - 2], LX = (long long)dims[rank - 1];
     unsigned long long O = batch;
     for (int d = 0; d < rank - 2; ++d) O *= dims[d];
-    const bool big = LY == 256 && LX == 256;
-    if ((big && O >= PLANE256_MIN_COUNT) || (plane_shape_supported(LY, LX) && O >= PLANE_MIN_COUNT)) {
+    const bool big = LY == 256 && LX == 256;  // no split-precision variant
+    if ((big && !plan.split && O >= PLANE256_MIN_COUNT) ||
+        (plane_shape_supported(LY, LX) && O >= PLANE_MIN_COUNT)) {
       PassDesc p;
       p.lg = ilog2c(LX);
       p.mode = MODE_PLANE;
@@ -518,11 +519,12 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       Prof& pf = P();
       const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
-      const int rc = (LY == 256 && LX == 256) ? launch_fft_xy256(xa, stream) : launch_fft_plane(LY, LX, xa, stream);
+      const int rc = (LY == 256 && LX == 256) ? launch_fft_xy256(xa, stream)
+                                               : launch_fft_plane(LY, LX, xa, stream, plan.split);
       if (prof) {
         HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
         char lab[64];
-        snprintf(lab, sizeof(lab), "fft_plane_f32_%dx%d", LY, LX);
+        snprintf(lab, sizeof(lab), "fft_plane_f32%s_%dx%d", plan.split ? "_split" : "", LY, LX);
         pf.recs.push_back({lab, pf.pool[pf.used], pf.pool[pf.used + 1]});
         pf.used += 2;
       }
@@ -774,6 +776,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
   HIP_TRY(hipMalloc(&mp->d_items, sizeof(MultiItem) * std::max<size_t>(1, mp->items.size())));
   HIP_TRY(hipMalloc(&mp->d_phase, sizeof(unsigned) * mp->phase_start.size()));
   HIP_TRY(hipMalloc(&mp->d_ctl, sizeof(MultiCtl)));
+  HIP_TRY(hipMemset(mp->d_ctl, 0, sizeof(MultiCtl)));  // checked before any launch is possible
   HIP_TRY(hipMemcpy(mp->d_passes, mp->passes.data(), sizeof(MultiPass) * mp->passes.size(), hipMemcpyHostToDevice));
   if (!mp->items.empty())
     HIP_TRY(hipMemcpy(mp->d_items, mp->items.data(), sizeof(MultiItem) * mp->items.size(), hipMemcpyHostToDevice));
@@ -783,7 +786,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
   mp->din = din;
   mp->dout = dout;
   mp->inverse = inverse;
-  mp->plane_ok = !mp->split && nphase == 3;
+  mp->plane_ok = nphase == 3;
   for (unsigned i = 0; i < count && mp->plane_ok; ++i)
     mp->plane_ok = trs[i].ps.size() == 3 && plane_shape_supported(dims3[3 * i + 1], dims3[3 * i + 2]);
   if (mp->plane_ok) {
@@ -828,6 +831,11 @@ int multi_exec(void* handle, void* stream) {
   }
   if (mp->nphase == 0) return 0;
   if (mp->mode == 2) {
+    // no grid barrier in this schedule: clear what an earlier launch reported
+    if (reset_multi_ctl(mp->d_ctl, stream)) {
+      set_error("multi-transform reset failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
     for (const PlaneGroup& g : mp->groups) {
       PlaneArgs a{};
       a.in = mp->din;
@@ -839,7 +847,7 @@ int multi_exec(void* handle, void* stream) {
       a.inverse = mp->inverse ? 1 : 0;
       a.tab = mp->d_tab + 2 * g.tab_off;
       a.ppt = g.ppt;
-      if (launch_fft_plane(g.ly, g.lx, a, stream)) {
+      if (launch_fft_plane(g.ly, g.lx, a, stream, mp->split)) {
         set_error("plane kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
         return -1;
       }
@@ -858,8 +866,9 @@ int multi_exec(void* handle, void* stream) {
       z.inverse = mp->inverse ? 1 : 0;
       z.tab = mp->d_tab_z + 2 * g.tab_off;
       if (sched_np((int)L) > 1 && get_stockham(PREC_F32, (int)L, &z.tw)) return -1;
+      // split precision applies to lines with a radix-16 stage, as in the line passes
       if (launch_fft_pass(PREC_F32, ilog2c(L), MODE_STRIDED, EPI_NONE, z,
-                          (unsigned long long)z.ncb * g.count, stream, false)) {
+                          (unsigned long long)z.ncb * g.count, stream, mp->split && L >= 16)) {
         set_error("z pass launch failed: %s", hipGetErrorString(hipGetLastError()));
         return -1;
       }

@@ -32,6 +32,8 @@ struct MultiItem {
 int launch_multi(const MultiPass* passes, const MultiItem* items, const unsigned* phase_start,
                  unsigned phase0, unsigned nphase, MultiCtl* ctl, unsigned grid, unsigned chunk,
                  unsigned spin_limit, void* stream, bool split);
+// Zero the control block (a kernel, so the call may be captured in a graph).
+int reset_multi_ctl(MultiCtl* ctl, void* stream);
 // Resident 256-thread workgroups per CU for the persistent kernel.
 int multi_max_blocks_per_cu(int* out, bool split);
 

@@ -139,6 +139,11 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
   }
 }
 
+int reset_multi_ctl(MultiCtl* ctl, void* stream) {
+  hipLaunchKernelGGL(k_multi_reset, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), ctl);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_multi(const MultiPass* passes, const MultiItem* items, const unsigned* phase_start,
                  unsigned phase0, unsigned nphase, MultiCtl* ctl, unsigned grid, unsigned chunk,
                  unsigned spin_limit, void* stream, bool split) {

@@ -29,9 +29,10 @@ struct PlaneArgs {
 int launch_fft_xy256(const PlaneArgs& a, void* stream);
 
 // LY x LX planes with 16 <= LY, LX <= 128 and LY * LX >= 1024, one workgroup
-// per plane holding it in LDS (fft_plane.hip).  0 on success, -1 for an
-// unsupported shape or grid, -2 on a launch error.  In place allowed.
-int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream);
+// per plane holding it in LDS (fft_plane.hip).  split: radix-16 stages on FP16
+// matrix cores with hi+lo operands.  0 on success, -1 for an unsupported
+// shape or grid, -2 on a launch error.  In place allowed.
+int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream, bool split = false);
 bool plane_shape_supported(long long ly, long long lx);
 
 }  // namespace fftamd

@@ -19,19 +19,22 @@
 
 namespace fftamd {
 
-template <int LY, int LX>
+template <int LY, int LX, bool SPLIT>
 struct PlaneGeom {
   static_assert(LY >= 16 && LX >= 16 && LY <= 128 && LX <= 128, "plane sides 16..128");
   using GX = Geom<LX, MODE_CONTIG, LY>;    // rows: LY lines of LX points
   using GY = Geom<LY, MODE_STRIDED, LX>;   // columns: LX lines of LY points
   static constexpr int NT = LY * LX / 16;
   static_assert(GX::NT == NT && GY::NT == NT, "16 points per thread in both phases");
-  static constexpr int LDS = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
+  static constexpr int LDS0 = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
+  // split-precision stages borrow the buffer as per-wave operand scratch
+  static constexpr int LDS = SPLIT && GX::SPLIT_ELEMS > LDS0 ? GX::SPLIT_ELEMS : LDS0;
 };
 
-template <int LY, int LX, bool INV>
+// SPLIT: radix-16 stages on FP16 matrix cores with hi+lo operands (fft_split.h).
+template <int LY, int LX, bool INV, bool SPLIT>
 __global__ void __launch_bounds__(LY * LX / 16) k_fft_plane(const PlaneArgs a) {
-  using PG = PlaneGeom<LY, LX>;
+  using PG = PlaneGeom<LY, LX, SPLIT>;
   using GX = typename PG::GX;
   using GY = typename PG::GY;
   __shared__ cpx<float> lds[PG::LDS];
@@ -62,7 +65,7 @@ __global__ void __launch_bounds__(LY * LX / 16) k_fft_plane(const PlaneArgs a) {
         v[b * RAD + r] = INV ? cswap(x) : x;
       }
     }
-    run_stages<float, LX, MODE_CONTIG, 0, false, LY>(v, lds, c, tau,
+    run_stages<float, LX, MODE_CONTIG, 0, SPLIT, LY>(v, lds, c, tau,
                                                      reinterpret_cast<const cpx<float>*>(a.tw_x));
     // last stage: point k = j + r*NS of row c -> column layout
     constexpr int NPX = GX::NP;
@@ -82,7 +85,7 @@ __global__ void __launch_bounds__(LY * LX / 16) k_fft_plane(const PlaneArgs a) {
     const int c = tid % LX, tau = tid / LX;
     stage_load_lds<float, LY, MODE_STRIDED, 0, LX>(v, lds, c, tau);
     __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
-    run_stages<float, LY, MODE_STRIDED, 0, false, LX>(v, lds, c, tau,
+    run_stages<float, LY, MODE_STRIDED, 0, SPLIT, LX>(v, lds, c, tau,
                                                       reinterpret_cast<const cpx<float>*>(a.tw_y));
     constexpr int NPY = GY::NP;
     constexpr int RL = sched_radix(LY, NPY - 1), NSL = sched_ns(LY, NPY - 1), NBL = 16 / RL;
@@ -106,37 +109,43 @@ bool plane_shape_supported(long long ly, long long lx) {
   return p2(ly) && p2(lx) && ly * lx >= 1024;
 }
 
-template <int LY, int LX>
+template <int LY, int LX, bool SPLIT>
 static void launch_shape(const PlaneArgs& a, hipStream_t s) {
-  constexpr int NT = PlaneGeom<LY, LX>::NT;
-  if (a.inverse) hipLaunchKernelGGL((k_fft_plane<LY, LX, true>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
-  else hipLaunchKernelGGL((k_fft_plane<LY, LX, false>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+  constexpr int NT = PlaneGeom<LY, LX, SPLIT>::NT;
+  if (a.inverse)
+    hipLaunchKernelGGL((k_fft_plane<LY, LX, true, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fft_plane<LY, LX, false, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
 }
 
-template <int LY>
+template <int LY, bool SPLIT>
 static bool dispatch_lx(int lx, const PlaneArgs& a, hipStream_t s) {
   switch (lx) {
-    case 16: if constexpr (LY * 16 >= 1024) { launch_shape<LY, 16>(a, s); return true; } else return false;
-    case 32: if constexpr (LY * 32 >= 1024) { launch_shape<LY, 32>(a, s); return true; } else return false;
-    case 64: launch_shape<LY, 64>(a, s); return true;
-    case 128: launch_shape<LY, 128>(a, s); return true;
+    case 16: if constexpr (LY * 16 >= 1024) { launch_shape<LY, 16, SPLIT>(a, s); return true; } else return false;
+    case 32: if constexpr (LY * 32 >= 1024) { launch_shape<LY, 32, SPLIT>(a, s); return true; } else return false;
+    case 64: launch_shape<LY, 64, SPLIT>(a, s); return true;
+    case 128: launch_shape<LY, 128, SPLIT>(a, s); return true;
     default: return false;
   }
 }
 
-int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream) {
+template <bool SPLIT>
+static bool dispatch_ly(int ly, int lx, const PlaneArgs& a, hipStream_t s) {
+  switch (ly) {
+    case 16: return dispatch_lx<16, SPLIT>(lx, a, s);
+    case 32: return dispatch_lx<32, SPLIT>(lx, a, s);
+    case 64: return dispatch_lx<64, SPLIT>(lx, a, s);
+    case 128: return dispatch_lx<128, SPLIT>(lx, a, s);
+    default: return false;
+  }
+}
+
+int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream, bool split) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (!plane_shape_supported(ly, lx) || a.nplanes == 0 || a.nplanes > 0xFFFFFFFFull / 1024 ||
       (a.tab && a.ppt == 0))
     return -1;
-  bool ok = false;
-  switch (ly) {
-    case 16: ok = dispatch_lx<16>(lx, a, s); break;
-    case 32: ok = dispatch_lx<32>(lx, a, s); break;
-    case 64: ok = dispatch_lx<64>(lx, a, s); break;
-    case 128: ok = dispatch_lx<128>(lx, a, s); break;
-    default: break;
-  }
+  const bool ok = split ? dispatch_ly<true>(ly, lx, a, s) : dispatch_ly<false>(ly, lx, a, s);
   if (!ok) return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

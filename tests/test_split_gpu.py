@@ -19,7 +19,8 @@ def announce(request):
 
 @pytest.mark.parametrize("shape,rank", [((16, 16, 16), 3), ((2, 64, 64, 64), 3), ((256, 256, 256), 3),
                                         ((32, 128, 64), 3), ((4096, 4096), 2), ((1 << 22,), 1),
-                                        ((8, 4096), 1), ((3, 512, 1024), 2)])
+                                        ((8, 4096), 1), ((3, 512, 1024), 2), ((4, 128, 128), 2),
+                                        ((2, 16, 128, 128), 3)])
 @pytest.mark.parametrize("inv", [False, True])
 def test_split_vs_oracle(gpu_lib, orc, shape, rank, inv):
     import torch
