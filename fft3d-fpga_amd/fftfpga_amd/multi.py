@@ -1,11 +1,17 @@
-"""Batches of 3D transforms of uneven power-of-two sizes in one persistent
-launch (fftamd_multi_* in include/fftfpga/fftfpga_hip.h), over torch tensors.
+"""Batches of 3D transforms of uneven power-of-two sizes as one plan
+(fftamd_multi_* in include/fftfpga/fftfpga_hip.h), over torch tensors.
 
     plan = MultiPlan(shapes, d_in, d_out, offsets_in, offsets_out, inverse=False)
     plan.tune(cache_path)          # optional: pick and cache the schedule
     plan()                         # enqueue on the current stream
     g = Graph([fwd, inv])          # hipGraph of a forward/inverse pair
     g()
+
+Schedules (plan.schedule()["mode"]): 0 one persistent launch with grid
+barriers between axes, 1 one persistent launch per axis, 2 per shape group a
+plane-kernel launch (x and y) and a line-pass launch (z); mode 2 exists when
+every transform has three axes of at least 16 points and x-y planes of at most
+128 x 128.
 """
 from __future__ import annotations
 
@@ -90,6 +96,8 @@ class MultiPlan:
                 "phases": ph.value}
 
     def set_schedule(self, mode: int, grid: int, chunk: int):
+        """mode 0, 1 or 2 (see the module doc); grid (resident workgroups) and
+        chunk (tiles per queue ticket) apply to the persistent modes."""
         if self._lib.fftamd_multi_set_schedule(self._h, mode, grid, chunk) != 0:
             raise ValueError(last_error())
 

@@ -138,6 +138,9 @@ int fftamd_profile_end(char *json_out, int cap);
  * column count, middle extent, middle bit-reversal bits, outer extent,
  * four-step modulus, column shift, source and destination buffer, columns
  * per workgroup, threads per workgroup) followed by one final-copy flag.
+ * Modes: 0 contiguous lines, 1 strided lines, 2 plane pass (the 2D transform
+ * of every plane of middle-extent rows x 2^log2-length columns; outer extent
+ * = planes, outer stride = plane size).
  * prec: 0 FP32, 1 FP64.  With out == NULL returns the pass count; otherwise
  * writes at most cap values and returns the pass count, or -1 on error. */
 int fftamd_describe_plan(int rank, const unsigned *dims, unsigned batch, unsigned flags, int prec,
