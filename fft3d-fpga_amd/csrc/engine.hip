@@ -551,7 +551,9 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
     a.out_o = p.out_o;
     a.ncol = p.ncol;
     const int L = 1 << p.lg;
-    const int C = geom_cols(L, p.mode);
+    // split mode applies to lines with a radix-16 stage (16..4096 points)
+    const bool split = plan.split && L >= 16 && L <= 4096;
+    const int C = geom_cols_for(L, p.mode, split);
     a.ncb = (p.ncol + C - 1) / C;
     a.K = p.K;
     a.kbits = p.kbits;
@@ -566,7 +568,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       a.ptw_cshift = p.cshift;
     }
     const unsigned long long per_outer = (unsigned long long)a.ncb * a.K;
-    const unsigned long long max_blocks = (0xFFFFFFFFULL / (unsigned)geom_threads(L, p.mode));
+    const unsigned long long max_blocks = (0xFFFFFFFFULL / (unsigned)geom_threads_for(L, p.mode, split));
     const unsigned long long outer_per_launch = std::max(1ULL, max_blocks / per_outer);
     for (unsigned long long o0 = 0; o0 < p.nouter; o0 += outer_per_launch) {
       const unsigned long long no = std::min(outer_per_launch, p.nouter - o0);
@@ -574,8 +576,6 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       Prof& pf = P();
       const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
-      // split mode applies to lines with a radix-16 stage (16..4096 points)
-      const bool split = plan.split && L >= 16 && L <= 4096;
       int rc = launch_fft_pass(plan.prec, p.lg, p.mode, p.epi, a, per_outer * no, stream, split);
       if (prof) {
         HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
@@ -861,7 +861,8 @@ int multi_exec(void* handle, void* stream) {
       z.in_p = z.out_p = s;
       z.in_c = z.out_c = 1;
       z.ncol = (unsigned)s;
-      z.ncb = (unsigned)((s + geom_cols((int)L, MODE_STRIDED) - 1) / geom_cols((int)L, MODE_STRIDED));
+      const int zc = geom_cols_for((int)L, MODE_STRIDED, mp->split && L >= 16);
+      z.ncb = (unsigned)((s + zc - 1) / zc);
       z.K = 1;
       z.inverse = mp->inverse ? 1 : 0;
       z.tab = mp->d_tab_z + 2 * g.tab_off;
@@ -1223,15 +1224,16 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
     set_error("describe_plan: buffer too small");
     return -1;
   }
+  auto sp = [&](const PassDesc& d) { return p.split && d.lg >= 4 && d.lg <= 12; };
   for (int i = 0; i < n; ++i) {
     const PassDesc& d = p.passes[i];
     long long* f = out + NF * i;
     long long v[NF] = {d.lg, d.mode, d.epi, d.transposing, d.in_p, d.out_p, d.in_c, d.out_c,
                        d.in_k, d.out_k, d.in_o, d.out_o, d.ncol, d.K, d.kbits,
                        (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
-                       d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K) : geom_cols(1 << d.lg, d.mode),
+                       d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K) : geom_cols_for(1 << d.lg, d.mode, sp(d)),
                        d.mode == MODE_PLANE ? (d.K == 256 ? 1024 : ((long long)d.K << d.lg) / 16)
-                                            : geom_threads(1 << d.lg, d.mode)};
+                                            : geom_threads_for(1 << d.lg, d.mode, sp(d))};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }
   out[NF * n] = p.final_copy ? 1 : 0;

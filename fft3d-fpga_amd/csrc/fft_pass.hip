@@ -18,12 +18,13 @@
 namespace fftamd {
 
 template <typename R, int L, int MODE, int EPI, bool SPLIT, int DIR>
-__global__ void __launch_bounds__(geom_threads(L, MODE))
+__global__ void __launch_bounds__(geom_threads_for(L, MODE, SPLIT))
     k_fft_pass(const PassArgs a) {
-  using G = Geom<L, MODE>;
+  constexpr int CC = geom_cols_for(L, MODE, SPLIT);
+  using G = Geom<L, MODE, CC>;
   constexpr bool NEED_LDS = G::NP > 1 || SPLIT;
   __shared__ cpx<R> lds[NEED_LDS ? (SPLIT ? G::LDS_ELEMS_SPLIT : G::LDS_ELEMS) : 1];
-  pass_block<R, L, MODE, EPI, SPLIT, DIR>(a, blockIdx.x, lds);
+  pass_block<R, L, MODE, EPI, SPLIT, DIR, CC>(a, blockIdx.x, lds);
 }
 
 // ---- launch table ------------------------------------------------------------
@@ -32,12 +33,11 @@ template <typename R, int MODE, int EPI, int LG, bool SPLIT>
 static void launch_one(const PassArgs& a, unsigned long long blocks, hipStream_t s) {
   constexpr int L = 1 << LG;
   // one kernel per direction: the direction swaps become register renames
+  constexpr int NT = geom_threads_for(L, MODE, SPLIT);
   if (a.inverse)
-    hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI, SPLIT, 1>), dim3((unsigned)blocks),
-                       dim3(geom_threads(L, MODE)), 0, s, a);
+    hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI, SPLIT, 1>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
   else
-    hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI, SPLIT, 0>), dim3((unsigned)blocks),
-                       dim3(geom_threads(L, MODE)), 0, s, a);
+    hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI, SPLIT, 0>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
 }
 
 template <typename R, int MODE, int EPI, int LG, bool SPLIT>

@@ -150,9 +150,9 @@ struct TileLoc {
   long long in_off, out_off;
 };
 
-template <int L, int MODE>
+template <int L, int MODE, int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ TileLoc tile_locate(const PassArgs& a, unsigned long long g) {
-  using G = Geom<L, MODE>;
+  using G = Geom<L, MODE, CC>;
   TileLoc t;
   const int tid = threadIdx.x;
   t.c = MODE == MODE_CONTIG ? tid / G::TPL : tid % G::C;
@@ -173,9 +173,9 @@ __device__ __forceinline__ TileLoc tile_locate(const PassArgs& a, unsigned long 
 
 // Raw stage-0 inputs of this thread straight from HBM (no arithmetic, so a
 // pipelined kernel can issue them one tile ahead).
-template <typename R, int L, int MODE>
+template <typename R, int L, int MODE, int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void tile_load(const PassArgs& a, const TileLoc& t, cpx<R>* v) {
-  using G = Geom<L, MODE>;
+  using G = Geom<L, MODE, CC>;
   constexpr int RAD = sched_radix(L, 0);
   constexpr int NB = G::PPT / RAD;
   const cpx<R>* __restrict__ in = reinterpret_cast<const cpx<R>*>(a.in) + t.in_off;
@@ -196,10 +196,10 @@ __device__ __forceinline__ void tile_load(const PassArgs& a, const TileLoc& t, c
 // whose raw inputs are in v.  DIR: 0 forward, 1 inverse (the swaps of
 // swap(DFT(swap(x))) are register renames), -1 taken from a.inverse at run
 // time (per-element selects).
-template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1>
+template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1, int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t, cpx<R>* v, cpx<R>* lds) {
   const bool inv = DIR < 0 ? a.inverse != 0 : DIR == 1;
-  using G = Geom<L, MODE>;
+  using G = Geom<L, MODE, CC>;
   constexpr int PPT = G::PPT, TPL = G::TPL;
   constexpr int NP = G::NP;
   const int c = t.c, tau = t.tau;
@@ -226,7 +226,7 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
     }
   }
 
-  run_stages<R, L, MODE, 0, SPLIT>(v, lds, c, tau, tw);
+  run_stages<R, L, MODE, 0, SPLIT, CC>(v, lds, c, tau, tw);
 
   // ---- last stage: point k = j + r*NS of the line is in v[b*RAD + r]
   constexpr int RL = sched_radix(L, NP - 1);
@@ -320,12 +320,13 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
 
 // One workgroup tile of a pass (block index g).  Used by k_fft_pass (one tile
 // per workgroup) and by the persistent multi-transform kernel.
-template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1>
+template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1,
+          int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds) {
-  cpx<R> v[Geom<L, MODE>::PPT];
-  const TileLoc t = tile_locate<L, MODE>(a, g);
-  tile_load<R, L, MODE>(a, t, v);
-  tile_finish<R, L, MODE, EPI, SPLIT, DIR>(a, t, v, lds);
+  cpx<R> v[Geom<L, MODE, CC>::PPT];
+  const TileLoc t = tile_locate<L, MODE, CC>(a, g);
+  tile_load<R, L, MODE, CC>(a, t, v);
+  tile_finish<R, L, MODE, EPI, SPLIT, DIR, CC>(a, t, v, lds);
 }
 
 }  // namespace fftamd

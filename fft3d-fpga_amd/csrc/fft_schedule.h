@@ -56,6 +56,12 @@ constexpr int geom_cols(int L, int mode) {
                              : (L >= 512 ? 32 : L >= 256 ? 16 : 256 / geom_tpl(L));
 }
 constexpr int geom_threads(int L, int mode) { return geom_cols(L, mode) * geom_tpl(L); }
+// Split-precision stages borrow LDS per wave: strided 512-point lines keep 16
+// columns there (32 measured slower with the larger operand scratch).
+constexpr int geom_cols_for(int L, int mode, bool split) {
+  return split && mode == MODE_STRIDED && L >= 512 ? 16 : geom_cols(L, mode);
+}
+constexpr int geom_threads_for(int L, int mode, bool split) { return geom_cols_for(L, mode, split) * geom_tpl(L); }
 
 // Largest line held by one workgroup (LDS budget).  Longer axes are split
 // four-step into two passes.

@@ -172,7 +172,8 @@ CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16]
          ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0), ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
          ([256, 256], 1, 4), ([2, 256, 256], 1, 4), ([4, 128, 128], 1, 0), ([128, 64], 3, 0),
          ([64, 256, 256], 1, 0), ([256, 256], 65, 0),
-         ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4)]
+         ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4), ([2, 512, 64], 1, 2),
+         ([4, 128, 128], 1, 2)]
 
 
 @pytest.mark.parametrize("dims,batch,flags", CASES)
