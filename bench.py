@@ -17,8 +17,9 @@ Further objects on the same line (default run): unfused_passes (the same
 steps with 256 x 256 planes as separate x and y passes), torch_fft_rocfft
 (torch.fft on the same data), batched (448 uneven transforms: persistent /
 grouped-plane launch against a per-transform loop), split_512 (split-precision
-512^3), autograd_16x128 (differentiable forward/backward), cpu_fftwf (threaded
-FFTW API on this host).
+512^3), autograd_16x128 (differentiable forward/backward), host_api_pcie (the
+drop-in call on host buffers: copy-in, transform, copy-out times), cpu_fftwf
+(threaded FFTW API on this host).
 """
 from __future__ import annotations
 
@@ -77,6 +78,8 @@ def parse():
                     help="skip the differentiable 16 x 128^3 forward/backward measurement")
     ap.add_argument("--no-cpu-fftw", action="store_true",
                     help="skip the threaded FFTW-API CPU reference (MKL's fftwf wrappers)")
+    ap.add_argument("--no-host-api", action="store_true",
+                    help="skip the drop-in host-buffer call (PCIe-inclusive timing)")
     return ap.parse_args()
 
 
@@ -237,6 +240,33 @@ def autograd_16x128(steps):
             "roofline_frac_forward": round(alg_f / (ms_f * 1e-3) / HBM_PEAK, 4),
             "roofline_frac_backward": round(alg_b / (ms_b * 1e-3) / HBM_PEAK, 4),
             "timing": "events on the current stream around the forward call and around backward()"}
+
+
+def host_api_pcie(N=256, seed=1, reps=3):
+    """The drop-in fftfpgaf_c2c_3d_ddr on pageable host buffers (numpy), as a
+    caller of the reference uses it: its fpga_t timings (HIP events around the
+    copy in, the transform and the copy out), median of `reps` calls."""
+    import numpy as np
+
+    import fftfpga_amd as fl
+    from fftfpga_amd import datagen
+    if fl.fpga_initialize("AMD HIP", fl.LIB_PATH, False) != 0:
+        return {"available": False, "reason": fl.last_error()}
+    x = datagen.uniform(N ** 3, seed)
+    out = np.empty_like(x)
+    fl.fftfpgaf_c2c_3d_ddr(N, x, out, False)  # plan, staging buffers
+    ts = []
+    for _ in range(reps):
+        t = fl.fftfpgaf_c2c_3d_ddr(N, x, out, False)
+        if not t.valid:
+            return {"available": False, "reason": fl.last_error()}
+        ts.append((t.pcie_write_t, t.exec_t, t.pcie_read_t))
+    w, e, r = (sorted(v)[len(v) // 2] for v in zip(*ts))
+    tot = w + e + r
+    return {"call": f"fftfpgaf_c2c_3d_ddr N={N} forward, pageable numpy buffers",
+            "pcie_write_ms": round(w, 3), "exec_ms": round(e, 4), "pcie_read_ms": round(r, 3),
+            "gflops_pcie_inclusive": round(flops_per_transform([N, N, N]) / (tot * 1e-3) / 1e9, 1),
+            "gflops_exec_only": round(flops_per_transform([N, N, N]) / (e * 1e-3) / 1e9, 1)}
 
 
 BATCH_COUNTS = {32: 256, 64: 160, 128: 32}
@@ -474,6 +504,8 @@ def main():
         result["split_512"] = split_512(max(3, min(args.steps, 10)))
     if not args.no_autograd:
         result["autograd_16x128"] = autograd_16x128(max(3, min(args.steps, 10)))
+    if not args.no_host_api:
+        result["host_api_pcie"] = host_api_pcie()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(dims, seed)
     if not args.no_cpu_fftw:
