@@ -104,6 +104,111 @@ __global__ void __launch_bounds__(LY * LX / 16) k_fft_plane(const PlaneArgs a) {
   }
 }
 
+// The 128 x 128 plane (128 KiB) in two halves: 512 threads hold the plane in
+// registers, 32 points each, and LDS stages half of it at a time (rows 0..63,
+// then 64..127; then columns 0..63, then 64..127).  LDS is then 68 KiB, so
+// two workgroups share a CU and one loads or stores while the other
+// computes; the whole-plane kernel above needs 136 KiB and runs one
+// workgroup per CU.
+template <int L, bool SPLIT>
+struct HalfGeom {
+  using GX = Geom<L, MODE_CONTIG, L / 2>;  // rows: L/2 lines of L points
+  using GY = Geom<L, MODE_STRIDED, L / 2>; // columns: L/2 lines of L points
+  static constexpr int NT = L * L / 32;
+  static_assert(GX::NT == NT && GY::NT == NT, "16 points per thread per half");
+  static constexpr int LDS0 = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
+  static constexpr int LDS = SPLIT && GX::SPLIT_ELEMS > LDS0 ? GX::SPLIT_ELEMS : LDS0;
+  // last row stage: radix 16, one butterfly per thread, so point k = tau + TPL r
+  static_assert(sched_radix(L, GX::NP - 1) == 16 && sched_ns(L, GX::NP - 1) == GX::TPL,
+                "points tau + TPL r: r < 8 are columns 0 .. L/2 - 1");
+};
+
+// WL, WS: the launch carries load or store weights.
+template <int L, bool INV, bool SPLIT, bool WL, bool WS>
+__global__ void __launch_bounds__(L * L / 32, 4) k_fft_plane_halves(const PlaneArgs a) {
+  using HG = HalfGeom<L, SPLIT>;
+  using GX = typename HG::GX;
+  using GY = typename HG::GY;
+  constexpr int H = L / 2;
+  __shared__ cpx<float> lds[HG::LDS];
+  const unsigned long long plane = blockIdx.x;
+  long long base = (long long)plane * a.plane_stride, obase = base;
+  if (a.tab) {  // planes of a batch of transforms placed anywhere
+    const unsigned t = (unsigned)(plane / a.ppt), z = (unsigned)(plane % a.ppt);
+    base = (long long)a.tab[2 * t] + (long long)z * (L * L);
+    obase = (long long)a.tab[2 * t + 1] + (long long)z * (L * L);
+  }
+  const cpx<float>* __restrict__ in = reinterpret_cast<const cpx<float>*>(a.in) + base;
+  cpx<float>* __restrict__ out = reinterpret_cast<cpx<float>*>(a.out) + obase;
+  const int tid = threadIdx.x;
+  cpx<float> rr[2][16];  // row results of row halves 0 and 1: point tau + TPL r of row c + H h
+
+  // ---- rows, in two halves: thread (row c, tau); stage 0 reads from HBM
+  const int rc = tid / GX::TPL, rtau = tid % GX::TPL;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    constexpr int RAD = sched_radix(L, 0), NB = 16 / RAD;
+    cpx<float>* v = rr[h];
+    const int row = rc + H * h;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = rtau + b * GX::TPL;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        const int e = row * L + j + r * (L / RAD);
+        cpx<float> x = in[e];
+        if (WL) x = cscale(x, a.w_load[(unsigned long long)(base + e) & a.w_mask]);
+        v[b * RAD + r] = INV ? cswap(x) : x;
+      }
+    }
+    if (h) __syncthreads();  // the first half's last stage has read the buffer
+    run_stages<float, L, MODE_CONTIG, 0, SPLIT, H>(v, lds, rc, rtau, reinterpret_cast<const cpx<float>*>(a.tw_x));
+  }
+
+  // ---- columns, in two halves: thread (column c, tau), adjacent threads on adjacent columns
+  const int cc = tid % H, ctau = tid / H;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();  // the previous stage has read the buffer
+    // points tau + TPL r with r in [8h, 8h + 8) are the columns of half h
+#pragma unroll
+    for (int rh = 0; rh < 2; ++rh)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) lds[GY::idx(rtau + GX::TPL * r, rc + H * rh)] = rr[rh][8 * h + r];
+    __syncthreads();
+    cpx<float> v[16];
+    stage_load_lds<float, L, MODE_STRIDED, 0, H>(v, lds, cc, ctau);
+    __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
+    run_stages<float, L, MODE_STRIDED, 0, SPLIT, H>(v, lds, cc, ctau, reinterpret_cast<const cpx<float>*>(a.tw_y));
+    constexpr int NPY = GY::NP;
+    constexpr int RL = sched_radix(L, NPY - 1), NSL = sched_ns(L, NPY - 1), NBL = 16 / RL;
+#pragma unroll
+    for (int b = 0; b < NBL; ++b) {
+      const int j = ctau + b * GY::TPL;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int e = (j + r * NSL) * L + H * h + cc;
+        cpx<float> x = v[b * RL + r];
+        if (INV) x = cswap(x);
+        if (WS) x = cscale(x, a.w_store[(unsigned long long)(obase + e) & a.w_mask]);
+        out[e] = x;
+      }
+    }
+  }
+}
+
+template <int L, bool INV, bool SPLIT>
+static void launch_halves(const PlaneArgs& a, hipStream_t s) {
+  const dim3 grid((unsigned)a.nplanes), block(HalfGeom<L, SPLIT>::NT);
+  if (a.w_load) {
+    if (a.w_store) hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, true, false>), grid, block, 0, s, a);
+  } else {
+    if (a.w_store) hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, false, false>), grid, block, 0, s, a);
+  }
+}
+
 bool plane_shape_supported(long long ly, long long lx) {
   auto p2 = [](long long v) { return v >= 16 && v <= 128 && (v & (v - 1)) == 0; };
   return p2(ly) && p2(lx) && ly * lx >= 1024;
@@ -112,10 +217,17 @@ bool plane_shape_supported(long long ly, long long lx) {
 template <int LY, int LX, bool SPLIT>
 static void launch_shape(const PlaneArgs& a, hipStream_t s) {
   constexpr int NT = PlaneGeom<LY, LX, SPLIT>::NT;
-  if (a.inverse)
-    hipLaunchKernelGGL((k_fft_plane<LY, LX, true, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_fft_plane<LY, LX, false, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+  // two workgroups per CU (FP32 stages; split-precision stages would spill
+  // registers at 512 threads and keep the whole-plane kernel)
+  if constexpr (LY == 128 && LX == 128 && !SPLIT) {
+    if (a.inverse) launch_halves<128, true, SPLIT>(a, s);
+    else launch_halves<128, false, SPLIT>(a, s);
+  } else {
+    if (a.inverse)
+      hipLaunchKernelGGL((k_fft_plane<LY, LX, true, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_fft_plane<LY, LX, false, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+  }
 }
 
 template <int LY, bool SPLIT>

@@ -280,41 +280,51 @@ def test_dft16_quarter(g):
 
 
 def emulate_plane256(plane):
-    """Replays k_fft_xy256's index maps for one plane (all four workgroups)."""
+    """Replays k_fft_xy256's index maps for one plane (all four workgroups of
+    512 threads): column stages per column half, row transforms per 32 rows."""
     out = np.full((256, 256), np.nan + 0j)
-    tid = np.arange(1024)
+    tid = np.arange(512)
     w256 = lambda j: np.exp(-2j * np.pi * (j % 256) / 256)  # noqa: E731
     r16 = np.arange(16)
+    x, g = tid & 127, tid >> 7
     for q in range(4):
-        # column first stage: lane (x, g), n1 = 4g + m, outputs k1 = 4q + u
-        x, g = tid & 255, tid >> 8
-        ybuf = np.full(16 * 4 * 256, np.nan + 0j)
-        for m in range(4):
-            n1 = 4 * g + m
-            v = plane[n1[:, None] + 16 * r16[None, :], x[:, None]]
-            o = _dft16_quarter(v, q)
-            for u in range(4):
-                idx = (4 * n1 + u) * 256 + x
-                assert np.isnan(ybuf[idx]).all()
-                ybuf[idx] = o[:, u]
-        assert not np.isnan(ybuf).any()
-        # column second stage: lane (x, u = g), k1 = 4q + u
-        u = g
-        v = ybuf[(4 * r16[None, :] + u[:, None]) * 256 + x[:, None]]
-        v = np.fft.fft(v * w256((4 * q + u)[:, None] * r16[None, :]), axis=1)
-        rows = np.full(64 * 272, np.nan + 0j)
-        for k2 in range(16):
-            idx = _row_idx(4 * k2 + u, x)
-            assert np.isnan(rows[idx]).all()
-            rows[idx] = v[:, k2]
-        # row transforms of the 64 local rows (the line-pass stages, checked
-        # separately), written to output row 16 k2 + 4q + u
-        pts = np.arange(256)
-        for r in range(64):
-            line = rows[_row_idx(r, pts)]
-            assert not np.isnan(line).any()
-            assert np.isnan(out[_grow(r, q)]).all()
-            out[_grow(r, q)] = np.fft.fft(line)
+        res = []
+        for h in range(2):
+            # column first stage: lane (x, g), n1 = 4g + m, outputs k1 = 4q + u
+            col = 128 * h + x
+            ybuf = np.full(16 * 4 * 128, np.nan + 0j)
+            for m in range(4):
+                n1 = 4 * g + m
+                rowe = (n1[:, None] + 16 * r16[None, :]) * 256
+                # buffer-load byte offsets stay inside the 512 KiB plane
+                assert (col[:, None] * 8 + rowe * 8).max() < 256 * 256 * 8
+                v = plane[n1[:, None] + 16 * r16[None, :], col[:, None]]
+                o = _dft16_quarter(v, q)
+                for u in range(4):
+                    idx = (4 * n1 + u) * 128 + x
+                    assert np.isnan(ybuf[idx]).all()
+                    ybuf[idx] = o[:, u]
+            assert not np.isnan(ybuf).any()
+            # column second stage: lane (x, u = g), k1 = 4q + u
+            u = g
+            v = ybuf[(4 * r16[None, :] + u[:, None]) * 128 + x[:, None]]
+            res.append(np.fft.fft(v * w256((4 * q + u)[:, None] * r16[None, :]), axis=1))
+        for j in range(2):
+            # lane (x, u) stores its points k2 = 8j .. 8j+7 of both halves as
+            # local rows 4 kk + u; then the line-pass stages run on the 32 rows
+            rows = np.full(32 * 272, np.nan + 0j)
+            u = g
+            for kk in range(8):
+                for h in range(2):
+                    idx = _row_idx(4 * kk + u, 128 * h + x)
+                    assert np.isnan(rows[idx]).all()
+                    rows[idx] = res[h][:, 8 * j + kk]
+            pts = np.arange(256)
+            for c in range(32):
+                line = rows[_row_idx(c, pts)]
+                assert not np.isnan(line).any()
+                assert np.isnan(out[_grow(32 * j + c, q)]).all()
+                out[_grow(32 * j + c, q)] = np.fft.fft(line)
     return out
 
 
@@ -324,11 +334,61 @@ def test_plane256_index_maps():
     got = emulate_plane256(plane)
     assert not np.isnan(got).any()
     assert np.allclose(got, np.fft.fft2(plane), rtol=1e-12, atol=1e-9)
-    # the 64 LDS rows fit the buffer and do not overlap
-    r, p = np.meshgrid(np.arange(64), np.arange(256), indexing="ij")
+    # 32 LDS rows fit the buffer and do not overlap; the buffer (the larger of
+    # the row block and the 16 x 4 x 128 first-stage block) lets two
+    # workgroups share a CU's 160 KiB
+    r, p = np.meshgrid(np.arange(32), np.arange(256), indexing="ij")
     idx = _row_idx(r, p)
-    assert idx.max() < 64 * 272 and np.unique(idx).size == idx.size
+    assert idx.max() < 32 * 272 and np.unique(idx).size == idx.size
+    assert 2 * max(32 * 272, 16 * 4 * 128) * 8 <= 160 * 1024
     assert sorted(_grow(r, q) for q in range(4) for r in range(64)) == list(range(256))
+
+
+def emulate_plane_halves(plane):
+    """Replays k_fft_plane_halves' index maps for one 128 x 128 plane (512
+    threads): row transforms of rows 0..63 then 64..127 held in registers,
+    then the columns in two halves of 64 through the strided LDS layout."""
+    L, H, TPL = 128, 64, 8
+    tid = np.arange(512)
+    rc, rtau = tid // TPL, tid % TPL
+    r16 = np.arange(16)
+    # after the row stages, thread (rc, rtau) holds point rtau + TPL r of its row
+    rr = [np.fft.fft(plane[rc + H * h], axis=1)[np.arange(512)[:, None], rtau[:, None] + TPL * r16[None, :]]
+          for h in range(2)]
+    out = np.full((L, L), np.nan + 0j)
+    cc, ctau = tid % H, tid // H
+    for h in range(2):
+        lds = np.full(L * (H + 1), np.nan + 0j)   # [y][column] with one pad column
+        for rh in range(2):
+            for r in range(8):
+                col, y = rtau + TPL * r, rc + H * rh
+                idx = y * (H + 1) + col
+                assert np.isnan(lds[idx]).all()
+                lds[idx] = rr[rh][:, 8 * h + r]
+        assert np.isnan(lds.reshape(L, H + 1)[:, H]).all()  # pad column unused
+        cols = lds.reshape(L, H + 1)[:, :H]
+        assert not np.isnan(cols).any()
+        # column stages (the line passes' strided stages), then point y of
+        # column cc goes to e = y * L + H h + cc
+        spec = np.fft.fft(cols, axis=0)
+        for t in range(512):
+            if ctau[t] == 0:
+                y = np.arange(L)
+                e = y * L + H * h + cc[t]
+                assert np.isnan(out.reshape(-1)[e]).all()
+                out.reshape(-1)[e] = spec[y, cc[t]]
+    return out
+
+
+def test_plane_halves_index_maps():
+    rng = np.random.default_rng(4)
+    plane = rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128))
+    got = emulate_plane_halves(plane)
+    assert not np.isnan(got).any()
+    assert np.allclose(got, np.fft.fft2(plane), rtol=1e-12, atol=1e-9)
+    # the larger of the 64-row block ([row][x + x/16]) and the 64-column block
+    # fits twice in a CU's 160 KiB
+    assert 2 * max(64 * (128 + 8), 128 * 65) * 8 <= 160 * 1024
 
 
 @pytest.mark.parametrize("nplanes", [1, 7, 8, 9, 256, 1000])
