@@ -4,10 +4,11 @@
 // A plane is 512 KiB, more than one CU's LDS, so four workgroups share it,
 // each owning 64 of the 256 output rows.  Column transform first: 256 =
 // 16 x 16, y = n1 + 16 n2, ky = k1 + 16 k2, and workgroup q owns the rows with
-// k1 in [4q, 4q+4) (ky = 16 k2 + 4q + u).  It reads the whole plane in
-// 512-byte row segments, evaluates only the four k1 outputs of each
-// first-stage 16-point DFT and only its four second-stage DFTs per column, so
-// no arithmetic is repeated across the four workgroups.  Its 64 rows then get
+// k1 = q mod 4 (k1 = q + 4u, ky = 16 k2 + 4u + q).  It reads the whole plane
+// in row segments, evaluates only the four k1 outputs of each first-stage
+// 16-point DFT and only its four second-stage DFTs per column, so no
+// arithmetic is repeated across the four workgroups; four outputs of one
+// residue class cost a third of the arithmetic of a whole 16-point DFT.  Its 64 rows then get
 // their 256-point row transforms (x) and are written as whole output rows.
 // The four workgroups of a plane are mapped to one XCD and dispatched back to
 // back (workgroups go to XCDs round-robin by index), so the plane comes from
@@ -44,7 +45,7 @@ static_assert(XY_NT == 4 * XY_XH, "column lanes: 128 columns x 4 groups");
 //      (4 n1 + u) * 128 + x (16 x 4 x 128 points; a wave reads or writes 64
 //      consecutive x);
 //  (2) 32 of the workgroup's rows, RowGeom layout [row][x + x/16] (local row
-//      r = 4 k2 + u holds output row ky = 16 k2 + 4q + u).
+//      r = 4 k2 + u holds output row ky = 16 k2 + 4u + q).
 constexpr int XY_LDS = RowGeom::LDS_ELEMS > 16 * 4 * XY_XH ? RowGeom::LDS_ELEMS : 16 * 4 * XY_XH;
 
 // Raw buffer resource over one plane (stride 0, byte range, 32-bit data
@@ -74,7 +75,7 @@ __device__ __forceinline__ float weight_load(__amdgpu_buffer_rsrc_t r, unsigned 
 }
 
 // output row of local row r of workgroup q
-__device__ __forceinline__ int xy_grow(int r, int q) { return 16 * (r >> 2) + 4 * q + (r & 3); }
+__device__ __forceinline__ int xy_grow(int r, int q) { return 16 * (r >> 2) + 4 * (r & 3) + q; }
 
 // x * (-i)^M
 template <int M>
@@ -86,16 +87,40 @@ __device__ __forceinline__ cpx<float> rot_negi(cpx<float> x) {
   else return {-x.y, x.x};
 }
 
-// Outputs 4 G .. 4 G + 3 of the 16-point DFT of v[] into o[0..3]: the second
-// half of dft16 evaluated for one output quarter only,
-// X[k1 + 4 G] = sum_n1 Y'[n1][k1] (-i)^{n1 G}.
+// x * W16^P, W16 = e^{-2 pi i / 16}
+template <int P>
+__device__ __forceinline__ cpx<float> mul_w16(cpx<float> x) {
+  constexpr int p = P & 15;
+  if constexpr ((p & 3) == 0) return rot_negi<p / 4>(x);
+  else if constexpr (p == 2) return mul_w8_1(x);
+  else if constexpr (p == 6) return mul_w8_3(x);
+  else if constexpr (p == 10) return rot_negi<2>(mul_w8_1(x));
+  else if constexpr (p == 14) return rot_negi<2>(mul_w8_3(x));
+  else {  // odd p: cos(pi p / 8) - i sin(pi p / 8) from cos(pi/8), sin(pi/8)
+    constexpr float c = (p == 1 || p == 15) ? K<float>::C16 : (p == 3 || p == 13) ? K<float>::S16
+                        : (p == 5 || p == 11) ? -K<float>::S16 : -K<float>::C16;
+    constexpr float sn = (p == 1 || p == 7) ? K<float>::S16 : (p == 3 || p == 5) ? K<float>::C16
+                         : (p == 9 || p == 15) ? -K<float>::S16 : -K<float>::C16;
+    return cmul(x, cpx<float>{c, -sn});
+  }
+}
+
+// Outputs G, G + 4, G + 8, G + 12 of the 16-point DFT of v[] into o[0..3].
+// With n = a + 4 b: X[G + 4u] = sum_a (-i)^{a u} W16^{a G} S_a, S_a = sum_b
+// v[a + 4 b] (-i)^{b G}: four 4-term sums with free rotations, three fixed
+// twiddles and one 4-point DFT.
 template <int G>
-__device__ __forceinline__ void dft16_quarter(cpx<float>* v, cpx<float>* o) {
-  dft16_head(v);
+__device__ __forceinline__ void dft16_residue(const cpx<float>* v, cpx<float>* o) {
+  cpx<float> s[4];
 #pragma unroll
-  for (int k1 = 0; k1 < 4; ++k1)
-    o[k1] = cadd(cadd(v[4 * k1], rot_negi<G>(v[4 * k1 + 1])),
-                 cadd(rot_negi<2 * G>(v[4 * k1 + 2]), rot_negi<3 * G>(v[4 * k1 + 3])));
+  for (int a = 0; a < 4; ++a)
+    s[a] = cadd(cadd(v[a], rot_negi<G>(v[a + 4])), cadd(rot_negi<2 * G>(v[a + 8]), rot_negi<3 * G>(v[a + 12])));
+  s[1] = mul_w16<G>(s[1]);
+  s[2] = mul_w16<2 * G>(s[2]);
+  s[3] = mul_w16<3 * G>(s[3]);
+  dft4(s);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) o[u] = s[u];
 }
 
 // v[r] *= W_256^{k r} from the radix-16 stage table (W^k, W^2k, W^4k, W^8k)
@@ -120,7 +145,7 @@ __device__ __forceinline__ void twiddle256(cpx<float>* v, int k, const cpx<float
   v[15] = cmul(v[15], cmul(w12, w3));
 }
 
-// Column transforms of column half H of the plane for row quarter Q: lane
+// Column transforms of column half H of the plane for the rows of workgroup Q: lane
 // (x, u) of the second stage returns the 16 points k2 = 0..15 of local rows
 // 4 k2 + u at column 128 H + x in res[].  INV: inverse transform as
 // swap(DFT(swap(x))), the swaps being register renames.  WL: inputs are
@@ -139,7 +164,7 @@ __device__ __forceinline__ void column_half(const PlaneArgs& a, const cpx<float>
   const unsigned voff = (unsigned)col * (unsigned)sizeof(cpx<float>);
   const __amdgpu_buffer_rsrc_t rw = plane_weights(a.w_load, (plane * a.plane_stride) & a.w_mask);
   // ---- first stage: for n1 = 4g .. 4g+3, the 16-point DFT over n2 of
-  // column col at rows n1 + 16 n2, four outputs k1 = 4Q + u
+  // column col at rows n1 + 16 n2, four outputs k1 = Q + 4u
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const int n1 = 4 * g + m;
@@ -152,7 +177,7 @@ __device__ __forceinline__ void column_half(const PlaneArgs& a, const cpx<float>
       v[n2] = INV ? cswap(s) : s;
     }
     cpx<float> o[4];
-    dft16_quarter<Q>(v, o);
+    dft16_residue<Q>(v, o);
 #pragma unroll
     for (int u = 0; u < 4; ++u) buf[(4 * n1 + u) * XY_XH + x] = o[u];
     // keep the scheduler from hoisting the next group's 16 loads above this
@@ -160,11 +185,11 @@ __device__ __forceinline__ void column_half(const PlaneArgs& a, const cpx<float>
     __builtin_amdgcn_sched_barrier(0);
   }
   __syncthreads();
-  // ---- second stage: lane (x, u = g): X[k1 + 16 k2] = DFT16_n1(Y[n1][k1] W256^{n1 k1}), k1 = 4Q + u
+  // ---- second stage: lane (x, u = g): X[k1 + 16 k2] = DFT16_n1(Y[n1][k1] W256^{n1 k1}), k1 = Q + 4u
   const int u = g;
 #pragma unroll
   for (int n1 = 0; n1 < 16; ++n1) res[n1] = buf[(4 * n1 + u) * XY_XH + x];
-  twiddle256(res, 4 * Q + u, reinterpret_cast<const cpx<float>*>(a.tw_y));
+  twiddle256(res, Q + 4 * u, reinterpret_cast<const cpx<float>*>(a.tw_y));
   dft16(res);
   __syncthreads();  // every first-stage value has been read before the buffer is reused
 }

@@ -253,30 +253,28 @@ def _row_idx(r, p):
 
 
 def _grow(r, q):
-    return 16 * (r >> 2) + 4 * q + (r & 3)
+    return 16 * (r >> 2) + 4 * (r & 3) + q
 
 
-def _dft16_quarter(v, g):
-    """dft16_head followed by the one-quarter output sum of fft_xy.hip."""
-    n2 = np.arange(4)
-    # head: Y[n1][k1] = sum_n2 v[4 n2 + n1] W4^{n2 k1}, then times W16^{n1 k1}
-    Y = np.zeros(v.shape[:-1] + (4, 4), complex)
-    for n1 in range(4):
-        for k1 in range(4):
-            Y[..., n1, k1] = (v[..., 4 * n2 + n1] * np.exp(-2j * np.pi * n2 * k1 / 4)).sum(-1)
-            Y[..., n1, k1] *= np.exp(-2j * np.pi * n1 * k1 / 16)
-    o = np.zeros(v.shape[:-1] + (4,), complex)
-    for k1 in range(4):
-        for n1 in range(4):
-            o[..., k1] += Y[..., n1, k1] * (-1j) ** ((n1 * g) % 4)
-    return o
+def _mul_w16(x, p):
+    return x * np.exp(-2j * np.pi * (p % 16) / 16)
+
+
+def _dft16_residue(v, g):
+    """fft_xy.hip's dft16_residue: outputs g, g+4, g+8, g+12 of a 16-point DFT
+    as four rotated 4-term sums, fixed twiddles W16^{a g}, one 4-point DFT."""
+    rot = lambda x, m: x * (-1j) ** (m % 4)  # noqa: E731
+    s = [v[..., a] + rot(v[..., a + 4], g) + rot(v[..., a + 8], 2 * g) + rot(v[..., a + 12], 3 * g)
+         for a in range(4)]
+    s = [_mul_w16(s[a], a * g) for a in range(4)]
+    return np.stack([sum(s[a] * (-1j) ** ((a * u) % 4) for a in range(4)) for u in range(4)], axis=-1)
 
 
 @pytest.mark.parametrize("g", range(4))
-def test_dft16_quarter(g):
+def test_dft16_residue(g):
     rng = np.random.default_rng(g)
     v = rng.standard_normal((5, 16)) + 1j * rng.standard_normal((5, 16))
-    assert np.allclose(_dft16_quarter(v, g), np.fft.fft(v, axis=1)[:, 4 * g:4 * g + 4], atol=1e-12)
+    assert np.allclose(_dft16_residue(v, g), np.fft.fft(v, axis=1)[:, g::4], atol=1e-12)
 
 
 def emulate_plane256(plane):
@@ -290,7 +288,7 @@ def emulate_plane256(plane):
     for q in range(4):
         res = []
         for h in range(2):
-            # column first stage: lane (x, g), n1 = 4g + m, outputs k1 = 4q + u
+            # column first stage: lane (x, g), n1 = 4g + m, outputs k1 = q + 4u
             col = 128 * h + x
             ybuf = np.full(16 * 4 * 128, np.nan + 0j)
             for m in range(4):
@@ -299,16 +297,16 @@ def emulate_plane256(plane):
                 # buffer-load byte offsets stay inside the 512 KiB plane
                 assert (col[:, None] * 8 + rowe * 8).max() < 256 * 256 * 8
                 v = plane[n1[:, None] + 16 * r16[None, :], col[:, None]]
-                o = _dft16_quarter(v, q)
+                o = _dft16_residue(v, q)
                 for u in range(4):
                     idx = (4 * n1 + u) * 128 + x
                     assert np.isnan(ybuf[idx]).all()
                     ybuf[idx] = o[:, u]
             assert not np.isnan(ybuf).any()
-            # column second stage: lane (x, u = g), k1 = 4q + u
+            # column second stage: lane (x, u = g), k1 = q + 4u
             u = g
             v = ybuf[(4 * r16[None, :] + u[:, None]) * 128 + x[:, None]]
-            res.append(np.fft.fft(v * w256((4 * q + u)[:, None] * r16[None, :]), axis=1))
+            res.append(np.fft.fft(v * w256((q + 4 * u)[:, None] * r16[None, :]), axis=1))
         for j in range(2):
             # lane (x, u) stores its points k2 = 8j .. 8j+7 of both halves as
             # local rows 4 kk + u; then the line-pass stages run on the 32 rows
