@@ -11,7 +11,8 @@
 // HBM instead of three, and the plane is read and written exactly once.
 //
 // The radix stages are the line passes' (fft_pass_dev.h), instantiated with a
-// whole plane's rows or columns per workgroup.
+// whole plane's rows or columns per workgroup.  128 x 128 FP32 planes use
+// k_fft_plane_halves below (two workgroups per CU).
 #include <hip/hip_runtime.h>
 
 #include "fft_pass_dev.h"
