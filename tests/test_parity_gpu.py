@@ -94,6 +94,40 @@ def test_3d_index_maps_exact(gpu_lib, N):
         assert np.abs(o).max() < 1e-3 * N ** 1.5
 
 
+@pytest.mark.parametrize("shape,rank,k", [
+    ((1024, 1024, 1024), 3, (3, 517, 1021)),   # 8 GiB; y and z four-step (32 x 32); byte offsets past 2^32
+    ((16384, 32768), 2, (9001, 17)),           # 4 GiB; x four-step 4 x 8192, y four-step 128 x 128
+    ((64, 256, 256, 256), 3, (255, 1, 128)),   # 8 GiB batch: 16384 planes of 256 x 256 in one plane pass
+])
+def test_large_arrays_single_tone(gpu_lib, shape, rank, k):
+    """Index maps past 32-bit element and byte offsets: a single complex
+    exponential at k lands on exactly that index with value prod(dims) (to
+    1e-4), and every other output stays below 1e-3 sqrt(prod(dims))."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    dims = shape[-rank:]
+    n = int(np.prod(dims))
+    tone = None
+    for a, (N, ka) in enumerate(zip(dims, k)):
+        e = torch.exp(2j * np.pi * ka * torch.arange(N, dtype=torch.float64, device="cuda") / N)
+        e = e.to(torch.complex64).reshape([N if b == a else 1 for b in range(rank)])
+        tone = e if tone is None else tone * e
+    x = tone.expand(shape).contiguous()
+    del tone
+    y = c2c_dev(x, rank)
+    del x
+    torch.cuda.synchronize()
+    flat = y.reshape(-1, n)
+    idx = int(np.ravel_multi_index(k, dims))
+    for b in range(flat.shape[0]):
+        assert int(torch.argmax(flat[b].abs())) == idx, b
+        assert abs(complex(flat[b, idx].item()) - n) < 1e-4 * n, b
+        flat[b, idx] = 0
+        assert flat[b].abs().max().item() < 1e-3 * n ** 0.5, b
+    del y, flat
+    torch.cuda.empty_cache()
+
+
 def test_3d_256_round_trip_and_parseval(gpu_lib):
     from fftfpga_amd import datagen
     N = 256

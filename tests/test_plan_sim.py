@@ -229,6 +229,30 @@ def test_plan_shapes_of_headline_configs(fl):
             assert q["NT"] <= 1024
 
 
+@pytest.mark.parametrize("dims,batch", [([1024, 1024, 1024], 1), ([16384, 32768], 1), ([256, 256, 256], 64)])
+def test_plan_bounds_of_large_arrays(fl, dims, batch):
+    """Arrays past 2^31 bytes (the GPU single-tone cases): every pass's first and
+    last line address stays inside the array (addresses are linear in the line
+    and point indices with non-negative strides, so the extremes bound them),
+    the grids fit the launch limits, and each pass covers the array once."""
+    passes, _ = describe(fl, dims, batch)
+    total = int(np.prod(dims)) * batch
+    assert total * 8 >= 1 << 32
+    for p in passes:
+        L = 1 << p["lg"]
+        if p["mode"] == 2:
+            assert p["nouter"] * p["in_o"] == total
+            continue
+        for s, o in (("in_p", "in_o"), ("out_p", "out_o")):
+            k, c = s.replace("_p", "_k"), s.replace("_p", "_c")
+            assert min(p[s], p[o], p[k], p[c]) >= 0
+            hi = (p["nouter"] - 1) * p[o] + (p["K"] - 1) * p[k] + (p["ncol"] - 1) * p[c] + (L - 1) * p[s]
+            assert hi < total, (p, s)
+        assert p["nouter"] * p["K"] * p["ncol"] * L == total
+        blocks = p["nouter"] * p["K"] * ((p["ncol"] + p["C"] - 1) // p["C"])
+        assert blocks * p["NT"] < 1 << 32
+
+
 @pytest.mark.parametrize("lg", range(1, 14))
 def test_bitrev_epilogue_runs(lg):
     """The bit-reversed epilogue writes point k = j + r*NS of the last stage to
