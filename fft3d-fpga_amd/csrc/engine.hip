@@ -263,10 +263,11 @@ static int split_strided(int prec, int lg, int* lg1, int* lg2) {
 
 // Fewest planes for which a plane pass replaces the separate x and y passes:
 // any count for planes up to 128 x 128 (one workgroup per plane, measured
-// faster from one 64 x 64 plane up); 64 for 256 x 256 planes (four workgroups
-// per plane; fewer leave CUs idle and measured slower than the line passes).
+// faster from one 64 x 64 plane up); 32 for 256 x 256 planes (four workgroups
+// per plane; with 16 or fewer planes the x and y line passes are faster,
+// measured 13.9 against 16.9 us at 16 planes, 20.6 against 18.1 at 32).
 static constexpr unsigned long long PLANE_MIN_COUNT = 1;
-static constexpr unsigned long long PLANE256_MIN_COUNT = 64;
+static constexpr unsigned long long PLANE256_MIN_COUNT = 32;
 
 static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec,
                       bool aliased, Plan& plan) {

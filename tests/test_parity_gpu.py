@@ -337,7 +337,7 @@ def test_device_api_orders_with_torch_default_stream(gpu_lib):
 
 # ---- fused 256 x 256 plane pass ------------------------------------------------------------------
 
-@pytest.mark.parametrize("shape,rank", [((64, 256, 256), 2), ((65, 256, 256), 2), ((2, 64, 256, 256), 3),
+@pytest.mark.parametrize("shape,rank", [((32, 256, 256), 2), ((64, 256, 256), 2), ((65, 256, 256), 2), ((2, 64, 256, 256), 3),
                                         ((4, 256, 256), 3), ((1, 128, 128), 2), ((3, 64, 128), 2),
                                         ((2, 128, 64), 3), ((16, 32, 32), 3), ((5, 16, 64), 2),
                                         ((128, 128, 128), 3), ((3, 64, 64, 64), 3)])

@@ -214,6 +214,9 @@ def test_plan_shapes_of_headline_configs(fl):
     # without plane fusion: three single passes (x contiguous, y and z strided)
     p, fc = describe(fl, [256, 256, 256], 1, flags=4)
     assert [q["mode"] for q in p] == [0, 1, 1] and not fc and all(q["dst"] == 1 for q in p)
+    # 256 x 256 planes go through the plane pass from 32 planes up
+    assert [q["mode"] for q in describe(fl, [256, 256], 32)[0]] == [2]
+    assert [q["mode"] for q in describe(fl, [256, 256], 16)[0]] == [0, 1]
     # double precision and split mode keep the separate passes
     assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, prec=1)[0]] == [0, 1, 1]
     assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, flags=2)[0]] == [0, 1, 1]
