@@ -69,7 +69,7 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 10 plain device copies of one workload-sized array (counter calibration)")
     ap.add_argument("--no-plane-fusion", action="store_true",
-                    help="run 256 x 256 planes as separate x and y passes (default: one fused pass)")
+                    help="run the x and y axes as separate line passes (default: one fused plane pass)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the uneven-size batched vs plain per-transform comparison")
     ap.add_argument("--no-split", action="store_true",

@@ -144,7 +144,8 @@ def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=No
     """Transform the last `rank` dims of a contiguous complex64/complex128 GPU
     tensor; leading dims are the batch.  Returns `out` (allocated if None).
     split=True selects the split-precision FP16 matrix-core radix-16 stages;
-    plane_fusion=False runs 256 x 256 planes as separate x and y passes."""
+    plane_fusion=False runs the two fastest axes as separate x and y line
+    passes instead of one plane pass."""
     import torch
     if not x.is_cuda:
         raise ValueError("c2c_dev needs a GPU tensor")
