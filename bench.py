@@ -13,6 +13,8 @@ roofline = the dominant kernel: algorithmic bytes of one launch (16 B per point:
          fraction for the whole transform (16 B per point per transform).
 cpu_baseline = the FP64 oracle (a scalar C port, 1 thread) on one forward
          transform of the same workload, timed on this host.
+accuracy = relative L2 error of this run's forward output (first transform)
+         against that oracle output, and of the round trip (y / N - x).
 Further objects on the same line (default run): unfused_passes (the same
 steps with 256 x 256 planes as separate x and y passes), torch_fft_rocfft
 (torch.fft on the same data), batched (448 uneven transforms: persistent /
@@ -83,22 +85,32 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(dims, seed):
-    """The oracle (FP64 C port, one thread) on one forward transform."""
+def cpu_baseline(dims, seed, gpu_forward=None, bitrev=False):
+    """The oracle (FP64 C port, one thread) on one forward transform.  With
+    gpu_forward (the GPU's forward output of the first transform of the same
+    seeded input, complex64), also the relative L2 error of that output against
+    the oracle's (bit-reversed order for the 1D drop-in)."""
     from oracle import oracle
     x = oracle.uniform(math.prod(dims), seed)
     oracle.fftnd(x, dims)  # warm-up (page-in)
     ts = []
     t_end = time.perf_counter() + 20.0
+    ref = None
     while len(ts) < 3 and (not ts or time.perf_counter() < t_end):
         t0 = time.perf_counter()
-        oracle.fftnd(x, dims)
+        ref = oracle.fftnd(x, dims)
         ts.append(time.perf_counter() - t0)
     t = sorted(ts)[len(ts) // 2]
-    return {"value": round(flops_per_transform(dims) / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{len(ts)} forward {'x'.join(map(str, dims))} transforms, FP64 oracle, "
-                      f"median {t * 1e3:.1f} ms"}
+    out = {"value": round(flops_per_transform(dims) / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+           "kind": "port",
+           "sample": f"{len(ts)} forward {'x'.join(map(str, dims))} transforms, FP64 oracle, "
+                     f"median {t * 1e3:.1f} ms"}
+    err = None
+    if gpu_forward is not None:
+        if bitrev:
+            ref = oracle.c2c_1d_expected(x, dims[0], 1)
+        err = oracle.rel_l2(gpu_forward, ref)
+    return out, err
 
 
 def cpu_fftwf(dims, seed):
@@ -471,6 +483,14 @@ def main():
         "kernels": {k: {"count": v["count"], "avg_us": round(1e3 * v["total_ms"] / v["count"], 3)}
                     for k, v in kernels.items()},
     }
+    # accuracy of this run's outputs: the first transform's forward output (checked
+    # against the FP64 oracle with the CPU baseline below) and the round trip
+    fwd_first = X.reshape(batch, -1)[0].cpu().numpy()
+    accuracy = {"bar": 2e-6}
+    if round_trip:
+        n_tot = math.prod(dims)
+        accuracy["rel_l2_round_trip"] = float(torch.linalg.vector_norm(y / n_tot - x) /
+                                              torch.linalg.vector_norm(x))
     if fusable and fusion["on"]:
         # the same K steps with the x and y axes as two separate passes
         fusion["on"] = False
@@ -507,7 +527,9 @@ def main():
     if not args.no_host_api:
         result["host_api_pcie"] = host_api_pcie()
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(dims, seed)
+        result["cpu_baseline"], err = cpu_baseline(dims, seed, fwd_first, bitrev)
+        accuracy["rel_l2_forward_vs_fp64_oracle"] = err
+    result["accuracy"] = accuracy
     if not args.no_cpu_fftw:
         result["cpu_fftwf"] = cpu_fftwf(dims, seed)
     print(json.dumps(result), flush=True)
