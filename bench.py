@@ -532,6 +532,10 @@ def main():
     result["accuracy"] = accuracy
     if not args.no_cpu_fftw:
         result["cpu_fftwf"] = cpu_fftwf(dims, seed)
+    result["cpu_reference_emulation"] = {
+        "available": False,
+        "reason": "the reference's CPU run (fft --num=64 --dim=3 --emulate) needs the Intel FPGA "
+                  "SDK for OpenCL emulator, which is not installed"}
     print(json.dumps(result), flush=True)
 
 
