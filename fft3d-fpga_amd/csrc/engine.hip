@@ -268,6 +268,10 @@ static int split_strided(int prec, int lg, int* lg1, int* lg2) {
 // measured 13.9 against 16.9 us at 16 planes, 20.6 against 18.1 at 32).
 static constexpr unsigned long long PLANE_MIN_COUNT = 1;
 static constexpr unsigned long long PLANE256_MIN_COUNT = 32;
+// Fewest 16-column groups for which the one-pass 4096-point column transform
+// replaces the four-step pair (four workgroups per group, as the 256 x 256
+// plane pass).
+static constexpr unsigned long long COL4096_MIN_GROUPS = 32;
 
 static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec,
                       bool aliased, Plan& plan) {
@@ -311,9 +315,34 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
     }
   }
 
+  // a 4096-point second-fastest axis in one pass (four workgroups per group of
+  // 16 columns, out of place), placed first so that the contiguous pass after
+  // it runs in place on the output
+  int col_axis = -1;
+  if (rank >= 2 && top == rank - 1 && prec == PREC_F32 && !plan.split && !(flags & FLAG_NO_PLANE_FUSION) &&
+      dims[rank - 2] == 4096) {
+    const long long LX = (long long)dims[rank - 1];
+    unsigned long long O = batch;
+    for (int d = 0; d < rank - 2; ++d) O *= dims[d];
+    if (col4096_supported(LX, O) && (unsigned long long)(LX / 16) * O >= COL4096_MIN_GROUPS) {
+      PassDesc p;
+      p.lg = 12;
+      p.mode = MODE_COL4096;
+      p.transposing = true;  // the four workgroups of a column group all read all of it
+      p.in_p = p.out_p = LX;
+      p.in_c = p.out_c = 1;
+      p.ncol = (unsigned)LX;
+      p.K = 1;
+      p.nouter = O;
+      p.in_o = p.out_o = 4096 * LX;
+      ps.push_back(p);
+      col_axis = rank - 2;
+    }
+  }
+
   for (int a = top; a >= 0; --a) {
     const long long L = (long long)dims[a];
-    if (L == 1) continue;
+    if (L == 1 || a == col_axis) continue;
     long long s = 1;
     for (int d = a + 1; d < rank; ++d) s *= (long long)dims[d];
     unsigned long long O = batch;
@@ -531,6 +560,37 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       }
       if (rc) {
         set_error("plane pass launch failed (rc=%d): %s", rc, hipGetErrorString(hipGetLastError()));
+        return -1;
+      }
+      cur = p.dst;
+      continue;
+    }
+    if (p.mode == MODE_COL4096) {
+      ColArgs ca{};
+      ca.in = ptr(p.src);
+      ca.out = ptr(p.dst);
+      ca.lx = p.in_p;
+      ca.in_o = p.in_o;
+      ca.ngroups = (unsigned)(p.ncol / 16);
+      ca.nitems = p.nouter * ca.ngroups;
+      if (get_stockham(PREC_F32, 256, &ca.tw)) return -1;
+      if (get_fourstep(PREC_F32, 4096, &ca.ptw_lo, &ca.ptw_hi)) return -1;
+      ca.ptw_hb = fourstep_hb(4096);
+      ca.w_load = pi == 0 ? w_load : nullptr;
+      ca.w_store = pi + 1 == plan.passes.size() ? w_store : nullptr;
+      ca.w_mask = (unsigned long long)plan.per - 1ULL;
+      ca.inverse = inverse ? 1 : 0;
+      Prof& pf = P();
+      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
+      const int rc = launch_fft_col4096(ca, stream);
+      if (prof) {
+        HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
+        pf.recs.push_back({"fft_col_f32_4096", pf.pool[pf.used], pf.pool[pf.used + 1]});
+        pf.used += 2;
+      }
+      if (rc) {
+        set_error("4096-point column pass launch failed (rc=%d): %s", rc, hipGetErrorString(hipGetLastError()));
         return -1;
       }
       cur = p.dst;
@@ -1226,15 +1286,22 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
     return -1;
   }
   auto sp = [&](const PassDesc& d) { return p.split && d.lg >= 4 && d.lg <= 12; };
+  // threads per workgroup of a plane pass: 512 for 256 x 256 (four per plane)
+  // and for FP32 128 x 128 (two halves), LY * LX / 16 otherwise
+  auto plane_threads = [&](const PassDesc& d) -> long long {
+    if (d.K == 256 || (d.K == 128 && d.lg == 7 && !p.split)) return 512;
+    return ((long long)d.K << d.lg) / 16;
+  };
   for (int i = 0; i < n; ++i) {
     const PassDesc& d = p.passes[i];
     long long* f = out + NF * i;
     long long v[NF] = {d.lg, d.mode, d.epi, d.transposing, d.in_p, d.out_p, d.in_c, d.out_c,
                        d.in_k, d.out_k, d.in_o, d.out_o, d.ncol, d.K, d.kbits,
                        (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
-                       d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K) : geom_cols_for(1 << d.lg, d.mode, sp(d)),
-                       d.mode == MODE_PLANE ? (d.K == 256 ? 1024 : ((long long)d.K << d.lg) / 16)
-                                            : geom_threads_for(1 << d.lg, d.mode, sp(d))};
+                       d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K)
+                       : d.mode == MODE_COL4096 ? 16 : geom_cols_for(1 << d.lg, d.mode, sp(d)),
+                       d.mode == MODE_PLANE ? plane_threads(d)
+                       : d.mode == MODE_COL4096 ? 512 : geom_threads_for(1 << d.lg, d.mode, sp(d))};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }
   out[NF * n] = p.final_copy ? 1 : 0;

@@ -35,4 +35,29 @@ int launch_fft_xy256(const PlaneArgs& a, void* stream);
 int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream, bool split = false);
 bool plane_shape_supported(long long ly, long long lx);
 
+// The 4096-point transform along the second-fastest axis of blocks of 4096
+// rows of lx elements (fft_col.hip): element (o, row, x) at o * in_o + row *
+// lx + x.  Four workgroups per group of 16 adjacent columns.
+struct ColArgs {
+  const void* in;
+  void* out;
+  long long lx;                  // row length in complex elements (multiple of 16)
+  long long in_o;                // elements between blocks (4096 * lx)
+  unsigned long long nitems;     // blocks x column groups
+  unsigned ngroups;              // column groups per block (lx / 16)
+  const void* tw;                // Stockham twiddles of 256-point lines
+  const void* ptw_lo;            // W4096^t = ptw_hi[t >> ptw_hb] * ptw_lo[t & (2^ptw_hb - 1)]
+  const void* ptw_hi;
+  unsigned ptw_hb;
+  const float* w_load;           // optional real weights (see PassArgs)
+  const float* w_store;
+  unsigned long long w_mask;
+  int inverse;
+};
+
+// 0 on success, -1 for an unsupported shape or grid, -2 on a launch error.
+// Out of place only: in and out must not overlap.
+int launch_fft_col4096(const ColArgs& a, void* stream);
+bool col4096_supported(long long lx, unsigned long long nouter);
+
 }  // namespace fftamd

@@ -12,8 +12,9 @@
 namespace fftamd {
 
 // MODE_PLANE: fused x+y pass over whole planes (fft_xy.hip for 256 x 256,
-// fft_plane.hip up to 128 x 128)
-enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1, MODE_PLANE = 2 };
+// fft_plane.hip up to 128 x 128); MODE_COL4096: the 4096-point second-fastest
+// axis in one pass (fft_col.hip)
+enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1, MODE_PLANE = 2, MODE_COL4096 = 3 };
 // EPI_TWIDDLE multiplies point k of column c by W_M^{(c >> cshift) * k}
 // (four-step split of a long axis); EPI_BITREV stores point k at bitrev(k).
 enum PassEpilogue { EPI_NONE = 0, EPI_TWIDDLE = 1, EPI_BITREV = 2 };

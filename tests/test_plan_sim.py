@@ -173,7 +173,7 @@ CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16]
          ([256, 256], 1, 4), ([2, 256, 256], 1, 4), ([4, 128, 128], 1, 0), ([128, 64], 3, 0),
          ([64, 256, 256], 1, 0), ([256, 256], 65, 0),
          ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4), ([2, 512, 64], 1, 2),
-         ([4, 128, 128], 1, 2)]
+         ([4, 128, 128], 1, 2), ([4096, 512], 1, 0), ([2, 4096, 256], 1, 0), ([4096, 16], 32, 0)]
 
 
 @pytest.mark.parametrize("dims,batch,flags", CASES)
@@ -193,7 +193,7 @@ def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
 @pytest.mark.parametrize("dims,batch,flags", [([64, 64, 64], 1, 0), ([2048, 32], 1, 0),
                                               ([1 << 14], 1, 1), ([1 << 14], 1, 0),
                                               ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
-                                              ([64, 256, 256], 1, 0)])
+                                              ([64, 256, 256], 1, 0), ([4096, 512], 1, 0)])
 def test_plan_replay_in_place(fl, orc, dims, batch, flags):
     passes, final_copy = describe(fl, dims, batch, flags, aliased=True)
     n = int(np.prod(dims)) * batch
@@ -223,9 +223,17 @@ def test_plan_shapes_of_headline_configs(fl):
     # 1D 2^22 bit-reversed: four-step, strided 512 then contiguous 8192
     p, fc = describe(fl, [1 << 22], 1, 1)
     assert [(q["lg"], q["mode"], q["epi"]) for q in p] == [(9, 1, 1), (13, 0, 2)]
-    # 2D 4096^2: contiguous 4096, then the strided axis split 64 x 64
+    # 2D 4096^2: the 4096-point column pass first (out of place), then the
+    # contiguous 4096-point rows in place on the output; no temporary buffer
     p, fc = describe(fl, [4096, 4096], 1)
+    assert [(q["lg"], q["mode"]) for q in p] == [(12, 3), (12, 0)] and not fc
+    assert [(q["src"], q["dst"]) for q in p] == [(0, 1), (1, 1)]
+    # without fused passes: contiguous 4096, then the strided axis split 64 x 64
+    p, fc = describe(fl, [4096, 4096], 1, flags=4)
     assert [(q["lg"], q["mode"]) for q in p] == [(12, 0), (6, 1), (6, 1)]
+    # fewer than 32 column groups keep the four-step pair
+    assert [q["mode"] for q in describe(fl, [4096, 256], 1)[0]] == [0, 1, 1]
+    assert [q["mode"] for q in describe(fl, [4096, 512], 1)[0]] == [3, 0]
     # every workgroup stays within 1024 threads
     for dims in ([2, 2, 2], [8192], [512, 512], [1 << 22]):
         for q in describe(fl, dims, 1)[0]:
@@ -414,6 +422,70 @@ def test_plane_halves_index_maps():
     # the larger of the 64-row block ([row][x + x/16]) and the 64-column block
     # fits twice in a CU's 160 KiB
     assert 2 * max(64 * (128 + 8), 128 * 65) * 8 <= 160 * 1024
+
+
+def emulate_col4096(block, lx):
+    """Replays k_fft_col4096's index maps for one 16-column group of a 4096 x lx
+    block (all four workgroups of 512 threads), with its byte offsets."""
+    out = np.full((4096, 16), np.nan + 0j)
+    tid = np.arange(512)
+    c16, gi = tid & 15, tid >> 4
+    r16 = np.arange(16)
+    lx8 = lx * 8
+    span = 4095 * lx8 + 16 * 8
+    w4096 = lambda t: np.exp(-2j * np.pi * (t % 4096) / 4096)  # noqa: E731
+    for q in range(4):
+        # first stage: thread (c16, gi), n1 = gi + 32 i, rows n1 + 256 n2
+        lines = {}
+        for i in range(8):
+            n1 = gi + 32 * i
+            voff = gi * lx8 + c16 * 8
+            soff = (32 * i + 256 * r16) * lx8
+            byte = voff[:, None] + soff[None, :]
+            assert byte.max() + 8 <= span and (byte % 8 == 0).all()
+            rows = byte // lx8
+            cols = (byte % lx8) // 8
+            assert np.array_equal(rows, n1[:, None] + 256 * r16[None, :])
+            v = block[rows, cols]
+            o = _dft16_residue(v, q)
+            for u in range(4):
+                for t in range(512):
+                    lines.setdefault((u, int(c16[t])), np.full(256, np.nan + 0j))[n1[t]] = o[t, u]
+        # stage 2 per half h: LDS lines c = 16 l + c16 (u = 2h + l), point n1 at n1 * 33 + c
+        for h in range(2):
+            lds = np.full(256 * 33, np.nan + 0j)
+            for l in range(2):
+                for c in range(16):
+                    idx = np.arange(256) * 33 + 16 * l + c
+                    assert np.isnan(lds[idx]).all()
+                    lds[idx] = lines[(2 * h + l, c)]
+            c, tau = tid % 32, tid // 32
+            k1 = q + 4 * (2 * h + (c >> 4))
+            for t in range(512):
+                if tau[t]:
+                    continue  # one thread per line replays the line
+                line = lds[np.arange(256) * 33 + c[t]]
+                assert not np.isnan(line).any()
+                spec = np.fft.fft(line * w4096(np.arange(256) * k1[t]))
+                for tt in range(16):  # thread tau stores k2 = tau + 16 r at row k1 + 16 k2
+                    voff = (k1[t] + 16 * tt) * lx8 + (c[t] & 15) * 8
+                    byte = voff + 256 * r16 * lx8
+                    assert byte.max() + 8 <= span
+                    rows, cols = byte // lx8, (byte % lx8) // 8
+                    assert np.isnan(out[rows, cols]).all()
+                    out[rows, cols] = spec[tt + 16 * r16]
+    return out
+
+
+@pytest.mark.parametrize("lx", [16, 64])
+def test_col4096_index_maps(lx):
+    rng = np.random.default_rng(lx)
+    block = rng.standard_normal((4096, 16)) + 1j * rng.standard_normal((4096, 16))
+    got = emulate_col4096(block, lx)
+    assert not np.isnan(got).any()
+    assert np.allclose(got, np.fft.fft(block, axis=0), rtol=1e-10, atol=1e-8)
+    # 32 lines of 256 points with one pad column fit twice in a CU's 160 KiB
+    assert 2 * 256 * 33 * 8 <= 160 * 1024
 
 
 @pytest.mark.parametrize("nplanes", [1, 7, 8, 9, 256, 1000])
