@@ -270,8 +270,9 @@ static constexpr unsigned long long PLANE_MIN_COUNT = 1;
 static constexpr unsigned long long PLANE256_MIN_COUNT = 32;
 // Fewest 16-column groups for which the one-pass 4096-point column transform
 // replaces the four-step pair (four workgroups per group, as the 256 x 256
-// plane pass).
-static constexpr unsigned long long COL4096_MIN_GROUPS = 32;
+// plane pass): measured 29.5 against 31.2 us at 32 groups (4096 x 512), 39.9
+// against 32.9 us at 64 (4096 x 1024), 137.6 against 103.3 us at 4096 x 4096.
+static constexpr unsigned long long COL4096_MIN_GROUPS = 64;
 
 static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec,
                       bool aliased, Plan& plan) {

@@ -341,9 +341,9 @@ def test_device_api_orders_with_torch_default_stream(gpu_lib):
                                         ((4, 256, 256), 3), ((1, 128, 128), 2), ((3, 64, 128), 2),
                                         ((2, 128, 64), 3), ((16, 32, 32), 3), ((5, 16, 64), 2),
                                         ((128, 128, 128), 3), ((3, 64, 64, 64), 3),
-                                        # the one-pass 4096-point column transform: 32 column
-                                        # groups as 1 x 512, 2 x 256 and 32 x 16 columns
-                                        ((4096, 512), 2), ((2, 4096, 256), 3), ((32, 4096, 16), 2)])
+                                        # the one-pass 4096-point column transform: 64 column
+                                        # groups as 1 x 1024, 4 x 256 and 64 x 16 columns
+                                        ((4096, 1024), 2), ((4, 4096, 256), 3), ((64, 4096, 16), 2)])
 def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
     """256 x 256 planes: 64, 65 (a partial XCD slot) and 128 planes, and 4
     (below the plane-pass threshold: line passes); planes up to 128 x 128
@@ -372,7 +372,7 @@ def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
 
 
 @pytest.mark.parametrize("shape,rank", [((64, 256, 256), 2), ((64, 256, 256), 3), ((2, 128, 128), 2),
-                                        ((4, 64, 64), 3), ((4096, 512), 2), ((2, 4096, 256), 3)])
+                                        ((4, 64, 64), 3), ((4096, 1024), 2), ((4, 4096, 256), 3)])
 def test_plane_pass_weights(gpu_lib, orc, shape, rank):
     """Weights applied in the fused pass's loads (both ranks) and stores (rank 2)."""
     import torch
@@ -400,14 +400,14 @@ def test_col4096_pass_unaligned_input(gpu_lib, orc):
     """The 4096-point column pass on an input that is only 8-byte aligned."""
     import torch
     from fftfpga_amd import c2c_dev
-    x = orc.uniform(4096 * 512, 45)
-    base = torch.zeros(4096 * 512 + 1, dtype=torch.complex64, device="cuda")
+    x = orc.uniform(4096 * 1024, 45)
+    base = torch.zeros(4096 * 1024 + 1, dtype=torch.complex64, device="cuda")
     base[1:] = torch.from_numpy(x).cuda()
-    xt = base[1:].view(4096, 512)
+    xt = base[1:].view(4096, 1024)
     assert xt.data_ptr() % 16 == 8
     y = c2c_dev(xt, 2)
     torch.cuda.synchronize()
-    assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [4096, 512])) <= TOL
+    assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [4096, 1024])) <= TOL
 
 
 def test_plane_pass_unaligned_input(gpu_lib, orc):

@@ -173,7 +173,7 @@ CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16]
          ([256, 256], 1, 4), ([2, 256, 256], 1, 4), ([4, 128, 128], 1, 0), ([128, 64], 3, 0),
          ([64, 256, 256], 1, 0), ([256, 256], 65, 0),
          ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4), ([2, 512, 64], 1, 2),
-         ([4, 128, 128], 1, 2), ([4096, 512], 1, 0), ([2, 4096, 256], 1, 0), ([4096, 16], 32, 0)]
+         ([4, 128, 128], 1, 2), ([4096, 1024], 1, 0), ([4, 4096, 256], 1, 0), ([4096, 16], 64, 0)]
 
 
 @pytest.mark.parametrize("dims,batch,flags", CASES)
@@ -193,7 +193,7 @@ def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
 @pytest.mark.parametrize("dims,batch,flags", [([64, 64, 64], 1, 0), ([2048, 32], 1, 0),
                                               ([1 << 14], 1, 1), ([1 << 14], 1, 0),
                                               ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
-                                              ([64, 256, 256], 1, 0), ([4096, 512], 1, 0)])
+                                              ([64, 256, 256], 1, 0), ([4096, 1024], 1, 0)])
 def test_plan_replay_in_place(fl, orc, dims, batch, flags):
     passes, final_copy = describe(fl, dims, batch, flags, aliased=True)
     n = int(np.prod(dims)) * batch
@@ -231,9 +231,9 @@ def test_plan_shapes_of_headline_configs(fl):
     # without fused passes: contiguous 4096, then the strided axis split 64 x 64
     p, fc = describe(fl, [4096, 4096], 1, flags=4)
     assert [(q["lg"], q["mode"]) for q in p] == [(12, 0), (6, 1), (6, 1)]
-    # fewer than 32 column groups keep the four-step pair
-    assert [q["mode"] for q in describe(fl, [4096, 256], 1)[0]] == [0, 1, 1]
-    assert [q["mode"] for q in describe(fl, [4096, 512], 1)[0]] == [3, 0]
+    # fewer than 64 column groups keep the four-step pair
+    assert [q["mode"] for q in describe(fl, [4096, 512], 1)[0]] == [0, 1, 1]
+    assert [q["mode"] for q in describe(fl, [4096, 1024], 1)[0]] == [3, 0]
     # every workgroup stays within 1024 threads
     for dims in ([2, 2, 2], [8192], [512, 512], [1 << 22]):
         for q in describe(fl, dims, 1)[0]:
