@@ -10,7 +10,7 @@ namespace fftamd {
 enum : unsigned {
   FLAG_BITREV_1D = 1u,   // leave 1D output bit-reversed within each transform
   FLAG_SPLIT_MFMA = 2u,  // radix-16 stages on FP16 MFMA with hi+lo operands
-  FLAG_NO_PLANE_FUSION = 4u,  // separate x and y passes over 256 x 256 planes
+  FLAG_NO_PLANE_FUSION = 4u,  // separate line passes instead of the plane and 4096-point column passes
 };
 
 struct Timing {

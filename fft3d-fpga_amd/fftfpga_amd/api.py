@@ -144,8 +144,9 @@ def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=No
     """Transform the last `rank` dims of a contiguous complex64/complex128 GPU
     tensor; leading dims are the batch.  Returns `out` (allocated if None).
     split=True selects the split-precision FP16 matrix-core radix-16 stages;
-    plane_fusion=False runs the two fastest axes as separate x and y line
-    passes instead of one plane pass."""
+    plane_fusion=False runs separate line passes instead of the fused ones
+    (the plane pass over the two fastest axes, the one-pass 4096-point column
+    transform)."""
     import torch
     if not x.is_cuda:
         raise ValueError("c2c_dev needs a GPU tensor")

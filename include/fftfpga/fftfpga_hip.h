@@ -29,9 +29,12 @@ extern "C" {
  * scaling and FP32 accumulation; other stages and twiddles stay FP32.  Single
  * precision only; lines of 16..4096 points use it. */
 #define FFTAMD_SPLIT_MFMA 0x2u
-/* Transforms whose two fastest axes are 256 x 256 run both axes in one fused
- * pass over each plane (a 3D transform then reads and writes HBM twice, not
- * three times).  This flag asks for separate x and y passes instead. */
+/* Single-precision transforms whose two fastest axes form planes up to
+ * 128 x 128, or 256 x 256 (from 32 planes up), run both axes in one fused pass
+ * over each plane (a 3D transform then reads and writes HBM twice, not three
+ * times); a second-fastest axis of 4096 points (from 64 groups of 16 columns
+ * up) runs in one pass instead of a four-step pair.  This flag asks for the
+ * separate line passes instead. */
 #define FFTAMD_NO_PLANE_FUSION 0x4u
 
 /* Single precision, rank 1..3, every dimension a power of two, `batch`
