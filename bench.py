@@ -15,7 +15,9 @@ cpu_baseline = the FP64 oracle (a scalar C port, 1 thread) on one forward
          transform of the same workload, timed on this host.
 accuracy = relative L2 error of this run's forward output (first transform)
          against that oracle output, and of the round trip (y / N - x).
-Further objects on the same line (default run): unfused_passes (the same
+Further objects on the same line (default run): plane_pass_path (the same
+steps with the 256 x 256 plane pass and a z pass instead of the y-split
+pair), unfused_passes (the same
 steps with 256 x 256 planes as separate x and y passes), torch_fft_rocfft
 (torch.fft on the same data), batched (448 uneven transforms: persistent /
 grouped-plane launch against a per-transform loop), split_512 (split-precision
@@ -72,6 +74,8 @@ def parse():
                     help="only run 10 plain device copies of one workload-sized array (counter calibration)")
     ap.add_argument("--no-plane-fusion", action="store_true",
                     help="run the x and y axes as separate line passes (default: one fused plane pass)")
+    ap.add_argument("--no-cube-split", action="store_true",
+                    help="256^3: the 256 x 256 plane pass and a z pass instead of the y-split pair")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the uneven-size batched vs plain per-transform comparison")
     ap.add_argument("--no-split", action="store_true",
@@ -395,13 +399,16 @@ def main():
     lib = fl.load()
 
     bitrev = rank == 1
-    fusion = {"on": not args.no_plane_fusion}
+    fusion = {"on": not args.no_plane_fusion, "cube": not args.no_cube_split}
     fusable = rank >= 2 and dims[-1] == 256 and dims[-2] == 256
+    cubic = rank == 3 and dims == [256, 256, 256]
 
     def step():
-        fl.c2c_dev(x, rank, inverse=False, out=X, bitrev_1d=bitrev, plane_fusion=fusion["on"])
+        fl.c2c_dev(x, rank, inverse=False, out=X, bitrev_1d=bitrev, plane_fusion=fusion["on"],
+                   cube_split=fusion["cube"])
         if round_trip:
-            fl.c2c_dev(X, rank, inverse=True, out=y, bitrev_1d=bitrev, plane_fusion=fusion["on"])
+            fl.c2c_dev(X, rank, inverse=True, out=y, bitrev_1d=bitrev, plane_fusion=fusion["on"],
+                       cube_split=fusion["cube"])
 
     for _ in range(args.warmup):
         step()
@@ -468,6 +475,7 @@ def main():
         "config": {"workload": args.workload, "dims": dims, "batch": batch,
                    "round_trip": round_trip, "placement": "out-of-place, HBM resident",
                    "plane_fusion": fusable and fusion["on"],
+                   "cube_split": cubic and fusion["on"] and fusion["cube"],
                    "inverse": "unnormalised",
                    "output_order": "bit-reversed per transform" if bitrev else "natural"},
         "hbm_gbs_algorithmic": round(gbs, 1),
@@ -491,6 +499,15 @@ def main():
         n_tot = math.prod(dims)
         accuracy["rel_l2_round_trip"] = float(torch.linalg.vector_norm(y / n_tot - x) /
                                               torch.linalg.vector_norm(x))
+    if cubic and fusion["on"] and fusion["cube"]:
+        # the same K steps through the 256 x 256 plane pass and a z pass
+        fusion["cube"] = False
+        for _ in range(args.warmup):
+            step()
+        ms_pl, _ = timed(False)
+        fusion["cube"] = True
+        result["plane_pass_path"] = {"ms_per_step": round(ms_pl, 5),
+                                     "cube_split_speedup": round(ms_pl / ms, 3)}
     if fusable and fusion["on"]:
         # the same K steps with the x and y axes as two separate passes
         fusion["on"] = False

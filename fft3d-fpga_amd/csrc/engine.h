@@ -11,6 +11,7 @@ enum : unsigned {
   FLAG_BITREV_1D = 1u,   // leave 1D output bit-reversed within each transform
   FLAG_SPLIT_MFMA = 2u,  // radix-16 stages on FP16 MFMA with hi+lo operands
   FLAG_NO_PLANE_FUSION = 4u,  // separate line passes instead of the plane and 4096-point column passes
+  FLAG_NO_CUBE_SPLIT = 8u,    // 256^3 blocks: the 256 x 256 plane pass and a z pass instead of the y-split pair
 };
 
 struct Timing {

@@ -100,6 +100,7 @@ struct State {
   std::map<PlanKey, Plan> plans;
   std::map<std::pair<int, int>, void*> tw_stockham;  // (prec, L)
   std::map<std::pair<int, long long>, std::pair<void*, void*>> tw_fourstep;  // (prec, M)
+  std::map<long long, void*> tw_direct;  // FP32 W_M^t, t = 0..M-1
 };
 
 static State& S() {
@@ -237,6 +238,27 @@ static int get_fourstep(int prec, long long M, const void** lo, const void** hi)
   return 0;
 }
 
+// FP32 table of W_M^t for every t < M (one load per twiddle)
+static int get_direct(long long M, const void** out) {
+  State& st = S();
+  auto it = st.tw_direct.find(M);
+  if (it == st.tw_direct.end()) {
+    std::vector<float> h(2 * (size_t)M);
+    for (long long t = 0; t < M; ++t) {
+      long double c, s;
+      twiddle_ld(t, M, &c, &s);
+      h[2 * t] = (float)c;
+      h[2 * t + 1] = (float)s;
+    }
+    void* d = nullptr;
+    HIP_TRY(hipMalloc(&d, h.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    it = st.tw_direct.emplace(M, d).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // planner: one or two passes per axis, last (contiguous) axis first
 // ---------------------------------------------------------------------------
@@ -290,11 +312,41 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   const bool bitrev = (flags & FLAG_BITREV_1D) && rank == 1;
   std::vector<PassDesc> ps;
 
+  // 256 x 256 x 256 blocks (the three fastest axes): two passes with the y
+  // axis split 64 x 4 between them -- x and y over b (out of place), then y
+  // over a and z (in place)
+  int top = rank - 1;
+  if (rank >= 3 && prec == PREC_F32 && !plan.split && !(flags & (FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT)) &&
+      dims[rank - 1] == 256 && dims[rank - 2] == 256 && dims[rank - 3] == 256) {
+    unsigned long long O = batch;
+    for (int d = 0; d < rank - 3; ++d) O *= dims[d];
+    PassDesc A;
+    A.lg = 8;
+    A.mode = MODE_CUBE_A;
+    A.transposing = true;  // residue a's output rows are other residues' input rows
+    A.in_p = A.out_p = 1;
+    A.in_c = A.out_c = 256;
+    A.ncol = 256;
+    A.K = 256;
+    A.nouter = O * 256;  // planes
+    A.in_o = A.out_o = 256 * 256;
+    ps.push_back(A);
+    PassDesc B;
+    B.lg = 8;
+    B.mode = MODE_CUBE_B;
+    B.in_p = B.out_p = 256 * 256;
+    B.in_c = B.out_c = 1;
+    B.ncol = 256 * 256;
+    B.nouter = O;  // blocks
+    B.in_o = B.out_o = 256LL * 256 * 256;
+    ps.push_back(B);
+    top = rank - 4;
+  }
+
   // planes of the two fastest axes in one fused x+y pass: 256 x 256 (four
   // workgroups per plane, out of place) or up to 128 x 128 (one workgroup
   // per plane in LDS, in place allowed)
-  int top = rank - 1;
-  if (rank >= 2 && prec == PREC_F32 && !(flags & FLAG_NO_PLANE_FUSION)) {
+  if (top == rank - 1 && rank >= 2 && prec == PREC_F32 && !(flags & FLAG_NO_PLANE_FUSION)) {
     const long long LY = (long long)dims[rank - 2], LX = (long long)dims[rank - 1];
     unsigned long long O = batch;
     for (int d = 0; d < rank - 2; ++d) O *= dims[d];
@@ -505,7 +557,7 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
 static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
                     Plan** out) {
   State& st = S();
-  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & (FLAG_BITREV_1D | FLAG_SPLIT_MFMA | FLAG_NO_PLANE_FUSION),
+  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & (FLAG_BITREV_1D | FLAG_SPLIT_MFMA | FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT),
               prec, aliased);
   auto it = st.plans.find(key);
   if (it == st.plans.end()) {
@@ -561,6 +613,36 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       }
       if (rc) {
         set_error("plane pass launch failed (rc=%d): %s", rc, hipGetErrorString(hipGetLastError()));
+        return -1;
+      }
+      cur = p.dst;
+      continue;
+    }
+    if (p.mode == MODE_CUBE_A || p.mode == MODE_CUBE_B) {
+      const bool A = p.mode == MODE_CUBE_A;
+      CubeArgs ca{};
+      ca.in = ptr(p.src);
+      ca.out = ptr(p.dst);
+      ca.nplanes = A ? p.nouter : p.nouter * 256;
+      if (get_stockham(PREC_F32, 256, &ca.tw256)) return -1;
+      if (get_direct(256, &ca.w256)) return -1;
+      // pass A is always the first pass of its plan; pass B takes the store
+      // weights when it is the last
+      ca.w_load = A && pi == 0 ? w_load : nullptr;
+      ca.w_store = !A && pi + 1 == plan.passes.size() ? w_store : nullptr;
+      ca.w_mask = (unsigned long long)plan.per - 1ULL;
+      ca.inverse = inverse ? 1 : 0;
+      Prof& pf = P();
+      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
+      const int rc = A ? launch_fft_cube_a(ca, stream) : launch_fft_cube_b(ca, stream);
+      if (prof) {
+        HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
+        pf.recs.push_back({A ? "fft_cube_a_f32_256" : "fft_cube_b_f32_256", pf.pool[pf.used], pf.pool[pf.used + 1]});
+        pf.used += 2;
+      }
+      if (rc) {
+        set_error("256^3 y-split pass launch failed (rc=%d): %s", rc, hipGetErrorString(hipGetLastError()));
         return -1;
       }
       cur = p.dst;
@@ -1155,6 +1237,8 @@ static void release_locked() {
     (void)hipFree(kv.second.second);
   }
   st.tw_fourstep.clear();
+  for (auto& kv : st.tw_direct) (void)hipFree(kv.second);
+  st.tw_direct.clear();
   if (st.d_in) (void)hipFree(st.d_in);
   if (st.d_out) (void)hipFree(st.d_out);
   if (st.h_stage) (void)hipHostFree(st.h_stage);
@@ -1300,9 +1384,13 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
                        d.in_k, d.out_k, d.in_o, d.out_o, d.ncol, d.K, d.kbits,
                        (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
                        d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K)
-                       : d.mode == MODE_COL4096 ? 16 : geom_cols_for(1 << d.lg, d.mode, sp(d)),
+                       : d.mode == MODE_COL4096 ? 16
+                       : d.mode == MODE_CUBE_A ? 64   // rows per workgroup
+                       : d.mode == MODE_CUBE_B ? 16   // columns per workgroup (x 4 rows x 256 planes)
+                       : geom_cols_for(1 << d.lg, d.mode, sp(d)),
                        d.mode == MODE_PLANE ? plane_threads(d)
-                       : d.mode == MODE_COL4096 ? 512 : geom_threads_for(1 << d.lg, d.mode, sp(d))};
+                       : d.mode == MODE_COL4096 || d.mode == MODE_CUBE_A || d.mode == MODE_CUBE_B ? 512
+                       : geom_threads_for(1 << d.lg, d.mode, sp(d))};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }
   out[NF * n] = p.final_copy ? 1 : 0;

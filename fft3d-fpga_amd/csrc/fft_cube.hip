@@ -1,0 +1,235 @@
+// fft_cube.hip -- 256 x 256 x 256 blocks in two passes over HBM with the y
+// axis split 64 x 4 between them, so that no tile is larger than one
+// workgroup's registers and LDS and no input is read twice.
+//
+// y = a + 4 b (a < 4, b < 64), ky = k + 64 d (k < 64, d < 4):
+//   X[k + 64 d] = sum_a W4^{a d} T_a[k],
+//   T_a[k]      = W256^{a k} sum_b x[a + 4 b] W64^{b k}.
+//
+// Pass A (k_fft_cube_a, one workgroup per plane and residue a): reads the 64
+// rows y = a + 4 b of a plane (128 KiB, whole 2 KiB rows), transforms each
+// along x (256 points), then each column along b (64 points), and writes the
+// column DFT (T_a[k] without its twiddle) to row 64 a + k.  Out of place: the
+// rows it writes are read by the other three residues.
+//
+// Pass B (k_fft_cube_b, one workgroup per (block, k, group of 16 columns)):
+// reads rows k + 64 a, a = 0..3, of 16 adjacent columns in all 256 planes
+// (128 KiB, 128-byte row segments), multiplies row k + 64 a by W256^{a k}, forms
+// X[k + 64 d] with a 4-point DFT over a, transforms the resulting 64 lines along z (256 points) and writes them
+// back to the same rows and columns (in place allowed).  A thread loads all
+// four rows of one column in 8 planes, so the 4-point DFT is in registers, and
+// sixteen adjacent lanes read or write one 128-byte row segment.  (Loading the
+// z transform's first-stage inputs of two rows per lane instead, with the
+// 4-point DFT finished across lane pairs by a DPP swap, saves one LDS exchange
+// but interleaves two rows over adjacent lanes: measured slower, 55 against
+// 48 us forward.)
+//
+// Against the 256 x 256 plane pass, whose four workgroups per plane each read
+// the whole 512 KiB plane, every tile here fits one workgroup: 512 threads
+// hold 32 points each in registers and LDS stages half a tile at a time
+// (68 KiB), so two workgroups share a CU.  Inverse: swap(DFT(swap(x))), the
+// swaps applied at pass A's loads and pass B's stores.
+#include <hip/hip_runtime.h>
+
+#include "fft_pass_dev.h"
+#include "fft_plane.h"
+
+namespace fftamd {
+
+constexpr int CB_L = 256;
+constexpr int CB_NT = 512;
+constexpr long long CB_PLANE = (long long)CB_L * CB_L;   // elements per plane
+constexpr long long CB_BLOCK = CB_PLANE * CB_L;          // elements per 256^3 block
+
+using CaRow = Geom<CB_L, MODE_CONTIG, 32>;     // pass A rows: 32 rows of 256, 16 threads each
+using CaCol = Geom<64, MODE_STRIDED, 128>;     // pass A columns: 128 columns of 64, 4 threads each
+using CbZ = Geom<CB_L, MODE_STRIDED, 32>;      // pass B: 32 lines (2 rows x 16 columns) of 256
+static_assert(CaRow::NT == CB_NT && CaCol::NT == CB_NT && CbZ::NT == CB_NT, "512 threads");
+static_assert(sched_np(CB_L) == 2 && sched_np(64) == 2, "two stages per line");
+constexpr int CA_LDS = CaRow::LDS_ELEMS > CaCol::LDS_ELEMS ? CaRow::LDS_ELEMS : CaCol::LDS_ELEMS;
+
+// Two workgroups per CU: at most 128 VGPRs (4 waves per SIMD), <= 68 KiB LDS each.
+template <bool INV, bool WL>
+__global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
+  __shared__ cpx<float> buf[CA_LDS];
+  __shared__ cpx<float> tws[16 * 15];  // row stage-1 twiddles W256^{k r} at [15 k + r - 1]
+  const unsigned long long plane = blockIdx.x >> 2;
+  const int res = blockIdx.x & 3;  // residue class a of the input rows
+  if (plane >= a.nplanes) return;  // uniform over the workgroup
+  const long long base = (long long)plane * CB_PLANE;
+  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
+  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
+  const int tid = threadIdx.x;
+  if (tid < 16 * 15) tws[tid] = wdir[(tid / 15) * (tid % 15 + 1)];  // read after the first barrier
+
+  // ---- x: rows b = 32 h + c (y = res + 4 b); thread (c, tau) ends with
+  // point kx = tau + 16 r of its row in r0[r] (h = 0) and r1[r] (h = 1)
+  const int c = tid >> 4, tau = tid & 15;
+  cpx<float> r0[16], r1[16];
+  // both halves' loads are issued first: the second half arrives while the
+  // first is transformed (r1 holds it either way, so this costs no registers)
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    cpx<float>* v = h ? r1 : r0;
+    const long long row = (long long)(res + 4 * (32 * h + c)) * CB_L;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const long long e = row + tau + 16 * r;
+      cpx<float> s = in[e];
+      if (WL) s = cscale(s, a.w_load[(unsigned long long)(base + e) & a.w_mask]);
+      v[r] = INV ? cswap(s) : s;
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the second half's loads ahead of the first half's arithmetic
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    cpx<float>* v = h ? r1 : r0;
+    stage_compute<float, CB_L, MODE_CONTIG, 0, false, 32>(v, tau, nullptr);  // radix 16, no twiddles
+    if (h) __syncthreads();  // every thread has read the first half's stage-1 inputs
+    stage_store_lds<float, CB_L, MODE_CONTIG, 0, 32>(v, buf, c, tau);
+    __syncthreads();
+    stage_load_lds<float, CB_L, MODE_CONTIG, 1, 32>(v, buf, c, tau);
+    // radix-16 stage over sub-transforms of span 16: twiddles W256^{tau r}
+    // from the LDS table (lanes of one tau read one address)
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * tau + r - 1]);
+    dft16(v);
+  }
+
+  // ---- y over b: columns kx = 128 hc + cc, 64-point lines; thread (cc, tq)
+  // ends with point k = tq + 4 r of its column
+  const int cc = tid & 127;
+  const int tq = __builtin_amdgcn_readfirstlane(tid >> 7);  // uniform over each wave
+#pragma unroll
+  for (int hc = 0; hc < 2; ++hc) {
+    __syncthreads();  // the buffer's previous contents have been read
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      buf[CaCol::idx(tau + 16 * r, c)] = r0[8 * hc + r];
+      buf[CaCol::idx(tau + 16 * r, 32 + c)] = r1[8 * hc + r];
+    }
+    __syncthreads();
+    cpx<float> v[16];
+    stage_load_lds<float, 64, MODE_STRIDED, 0, 128>(v, buf, cc, tq);
+    __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
+    stage_compute<float, 64, MODE_STRIDED, 0, false, 128>(v, tq, nullptr);  // four radix-4, no twiddles
+    stage_store_lds<float, 64, MODE_STRIDED, 0, 128>(v, buf, cc, tq);
+    __syncthreads();
+    stage_load_lds<float, 64, MODE_STRIDED, 1, 128>(v, buf, cc, tq);
+    // radix-16 stage over sub-transforms of span 4: twiddles W64^{tq r} =
+    // W256^{4 tq r}, the same for the whole wave (scalar loads, no products)
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[4 * tq * r]);
+    dft16(v);
+    // column DFT point k = tq + 4 r to row 64 res + k (pass B applies W256^{res k})
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      out[(long long)(64 * res + tq + 4 * r) * CB_L + 128 * hc + cc] = v[r];
+  }
+}
+
+template <bool INV, bool WS>
+__global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
+  __shared__ cpx<float> buf[CbZ::LDS_ELEMS];
+  const unsigned b = blockIdx.x;
+  const int xg = b & 15;               // group of 16 columns
+  const int k = (b >> 4) & 63;         // rows k + 64 a
+  const unsigned long long blk = b >> 10;
+  if (blk >= a.nplanes / CB_L) return;  // uniform over the workgroup
+  const long long base = (long long)blk * CB_BLOCK + (long long)k * CB_L + 16 * xg;
+  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
+  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  const cpx<float>* __restrict__ tw256 = reinterpret_cast<const cpx<float>*>(a.tw256);
+  const int tid = threadIdx.x;
+
+  // ---- y over a: thread (x, zs) holds planes z = zs + 32 j; t[j][d] = X[k + 64 d].
+  // Sixteen adjacent lanes read one 128-byte row segment.
+  const int x = tid & 15, zs = tid >> 4;
+  cpx<float> t[8][4];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const long long zo = (long long)(zs + 32 * j) * CB_PLANE + x;
+#pragma unroll
+    for (int aa = 0; aa < 4; ++aa) t[j][aa] = in[zo + 64 * CB_L * aa];
+  }
+  // T_a[k] = W256^{a k} times pass A's column DFT (k is uniform over the workgroup)
+  const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
+  const cpx<float> w1 = wdir[k], w2 = wdir[2 * k], w3 = wdir[3 * k];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    t[j][1] = cmul(t[j][1], w1);
+    t[j][2] = cmul(t[j][2], w2);
+    t[j][3] = cmul(t[j][3], w3);
+    dft4(t[j]);
+  }
+
+  // ---- z: the 32 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 16 (d - 2 e) + x
+  const int cz = tid & 31;
+  const int tz = tid >> 5;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    if (e) __syncthreads();  // the first half's last stage has read the buffer
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      buf[CbZ::idx(x, zs + 32 * j)] = t[j][2 * e];
+      buf[CbZ::idx(16 + x, zs + 32 * j)] = t[j][2 * e + 1];
+    }
+    __syncthreads();
+    cpx<float> v[16];
+    stage_load_lds<float, CB_L, MODE_STRIDED, 0, 32>(v, buf, cz, tz);
+    __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
+    run_stages<float, CB_L, MODE_STRIDED, 0, false, 32>(v, buf, cz, tz, tw256);
+    // point kz = tz + 16 r of row k + 64 d, column 16 xg + (cz & 15)
+    const long long ob = (long long)(64 * (2 * e + (cz >> 4))) * CB_L + (cz & 15);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const long long o = ob + (long long)(tz + 16 * r) * CB_PLANE;
+      cpx<float> s = INV ? cswap(v[r]) : v[r];
+      if (WS) s = cscale(s, a.w_store[(unsigned long long)(base + o) & a.w_mask]);
+      // non-temporal: measured 2% faster over a forward + inverse round trip
+      // (49.9 -> 47.9 us for this pass; non-temporal loads in pass A were slower)
+      __builtin_nontemporal_store(__builtin_bit_cast(long long, s), reinterpret_cast<long long*>(out + o));
+    }
+  }
+}
+
+static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsigned long long* blocks) {
+  const unsigned long long b = nplanes * per;
+  if (b > 0xFFFFFFFFull / CB_NT) return false;  // grid size in work-items is 32-bit
+  *blocks = b;
+  return true;
+}
+
+int launch_fft_cube_a(const CubeArgs& a, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  unsigned long long blocks;
+  if (a.nplanes == 0 || !cube_grid(a.nplanes, 4, &blocks) || a.w_store) return -1;
+  const dim3 g((unsigned)blocks), t(CB_NT);
+  if (a.inverse) {
+    if (a.w_load) hipLaunchKernelGGL((k_fft_cube_a<true, true>), g, t, 0, s, a);
+    else hipLaunchKernelGGL((k_fft_cube_a<true, false>), g, t, 0, s, a);
+  } else {
+    if (a.w_load) hipLaunchKernelGGL((k_fft_cube_a<false, true>), g, t, 0, s, a);
+    else hipLaunchKernelGGL((k_fft_cube_a<false, false>), g, t, 0, s, a);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_fft_cube_b(const CubeArgs& a, void* stream) {
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  unsigned long long blocks;
+  if (a.nplanes == 0 || a.nplanes % CB_L || !cube_grid(a.nplanes / CB_L, 1024, &blocks) || a.w_load)
+    return -1;
+  const dim3 g((unsigned)blocks), t(CB_NT);
+  if (a.inverse) {
+    if (a.w_store) hipLaunchKernelGGL((k_fft_cube_b<true, true>), g, t, 0, s, a);
+    else hipLaunchKernelGGL((k_fft_cube_b<true, false>), g, t, 0, s, a);
+  } else {
+    if (a.w_store) hipLaunchKernelGGL((k_fft_cube_b<false, true>), g, t, 0, s, a);
+    else hipLaunchKernelGGL((k_fft_cube_b<false, false>), g, t, 0, s, a);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace fftamd

@@ -13,8 +13,10 @@ namespace fftamd {
 
 // MODE_PLANE: fused x+y pass over whole planes (fft_xy.hip for 256 x 256,
 // fft_plane.hip up to 128 x 128); MODE_COL4096: the 4096-point second-fastest
-// axis in one pass (fft_col.hip)
-enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1, MODE_PLANE = 2, MODE_COL4096 = 3 };
+// axis in one pass (fft_col.hip); MODE_CUBE_A / MODE_CUBE_B: the two passes of
+// 256 x 256 x 256 blocks with the y axis split 64 x 4 (fft_cube.hip)
+enum PassMode { MODE_CONTIG = 0, MODE_STRIDED = 1, MODE_PLANE = 2, MODE_COL4096 = 3, MODE_CUBE_A = 4,
+                MODE_CUBE_B = 5 };
 // EPI_TWIDDLE multiplies point k of column c by W_M^{(c >> cshift) * k}
 // (four-step split of a long axis); EPI_BITREV stores point k at bitrev(k).
 enum PassEpilogue { EPI_NONE = 0, EPI_TWIDDLE = 1, EPI_BITREV = 2 };

@@ -8,6 +8,7 @@ device-pointer helpers for torch tensors.
 from ._lib import LIB_PATH, SIGNATURES, FpgaT, last_error, load
 from .api import (
     FFTAMD_BITREV_1D,
+    FFTAMD_NO_CUBE_SPLIT,
     FFTAMD_NO_PLANE_FUSION,
     FFTAMD_SPLIT_MFMA,
     c2c_dev,

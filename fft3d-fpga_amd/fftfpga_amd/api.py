@@ -21,6 +21,7 @@ from ._lib import FpgaT, last_error, load
 FFTAMD_BITREV_1D = 0x1
 FFTAMD_SPLIT_MFMA = 0x2
 FFTAMD_NO_PLANE_FUSION = 0x4
+FFTAMD_NO_CUBE_SPLIT = 0x8
 
 
 def _ptr(a, dtype, need: int):
@@ -140,13 +141,15 @@ def _stream_handle(t) -> ctypes.c_void_p:
 
 
 def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=None,
-            split: bool = False, plane_fusion: bool = True):
+            split: bool = False, plane_fusion: bool = True, cube_split: bool = True):
     """Transform the last `rank` dims of a contiguous complex64/complex128 GPU
     tensor; leading dims are the batch.  Returns `out` (allocated if None).
     split=True selects the split-precision FP16 matrix-core radix-16 stages;
     plane_fusion=False runs separate line passes instead of the fused ones
     (the plane pass over the two fastest axes, the one-pass 4096-point column
-    transform)."""
+    transform, the y-split pair of 256^3 blocks); cube_split=False keeps the
+    256 x 256 plane pass and a z pass for 256^3 blocks instead of the y-split
+    pair."""
     import torch
     if not x.is_cuda:
         raise ValueError("c2c_dev needs a GPU tensor")
@@ -169,7 +172,7 @@ def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=No
     cd = (ctypes.c_uint * rank)(*dims)
     fn = load().fftamd_c2c_dev if x.dtype == torch.complex64 else load().fftamd_z2z_dev
     flags = ((FFTAMD_BITREV_1D if bitrev_1d else 0) | (FFTAMD_SPLIT_MFMA if split else 0)
-             | (0 if plane_fusion else FFTAMD_NO_PLANE_FUSION))
+             | (0 if plane_fusion else FFTAMD_NO_PLANE_FUSION) | (0 if cube_split else FFTAMD_NO_CUBE_SPLIT))
     rc = fn(rank, cd, batch, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
             1 if inverse else 0, flags, _stream_handle(x))
     if rc != 0:

@@ -372,9 +372,12 @@ def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
 
 
 @pytest.mark.parametrize("shape,rank", [((64, 256, 256), 2), ((64, 256, 256), 3), ((2, 128, 128), 2),
-                                        ((4, 64, 64), 3), ((4096, 1024), 2), ((4, 4096, 256), 3)])
+                                        ((4, 64, 64), 3), ((4096, 1024), 2), ((4, 4096, 256), 3),
+                                        ((256, 256, 256), 3), ((2, 256, 256, 256), 3)])
 def test_plane_pass_weights(gpu_lib, orc, shape, rank):
-    """Weights applied in the fused pass's loads (both ranks) and stores (rank 2)."""
+    """Weights applied in the fused pass's loads (both ranks) and stores (rank 2);
+    256^3 blocks: load weights in the y-split pair's first pass, store weights
+    in its second."""
     import torch
     from fftfpga_amd.autograd import weighted_transform
     n = int(np.prod(shape))
@@ -423,3 +426,66 @@ def test_plane_pass_unaligned_input(gpu_lib, orc):
     y = c2c_dev(xt, 2)
     torch.cuda.synchronize()
     assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [256, 256], batch=64)) <= TOL
+
+
+# ---- 256^3 blocks: two passes with the y axis split 64 x 4 -----------------------------------------
+
+@pytest.mark.parametrize("shape", [(256, 256, 256), (2, 256, 256, 256)])
+def test_cube_split_vs_oracle(gpu_lib, orc, shape):
+    """The y-split pair (x and 64-point y over rows y = a + 4b; then the 4-point
+    y stage and z) against the FP64 oracle and against the plane-pass path,
+    out of place and in place, both directions, one block and a batch of two."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    n = int(np.prod(shape))
+    x = orc.uniform(n, 4100 + len(shape))
+    batch = n // (1 << 24)
+    xt = torch.from_numpy(x.reshape(shape)).cuda()
+    for inv in (False, True):
+        ref = orc.fftnd(x, [256, 256, 256], batch=batch, inverse=inv)
+        y = c2c_dev(xt, 3, inverse=inv)
+        yp = c2c_dev(xt, 3, inverse=inv, cube_split=False)
+        z = xt.clone()
+        c2c_dev(z, 3, inverse=inv, out=z)
+        torch.cuda.synchronize()
+        err = orc.rel_l2(y.cpu().numpy(), ref)
+        errp = orc.rel_l2(yp.cpu().numpy(), ref)
+        errz = orc.rel_l2(z.cpu().numpy(), ref)
+        print(f"cube split {shape} inv={inv} split={err:.3e} plane={errp:.3e} inplace={errz:.3e}")
+        assert err <= TOL and errp <= TOL and errz <= TOL
+    assert np.array_equal(xt.cpu().numpy().reshape(-1), x)  # input untouched
+
+
+def test_cube_split_single_tones_exact(gpu_lib):
+    """Single complex exponentials land exactly on their bins through the
+    y-split pair (every y residue and output row block, both directions)."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    N = 256
+    i = np.arange(N, dtype=np.int64)
+    for (kz, ky, kx) in [(0, 0, 0), (1, 2, 3), (255, 64, 128), (17, 191, 5), (128, 255, 254)]:
+        ph = (kz * i[:, None, None] + ky * i[None, :, None] + kx * i[None, None, :]) % N
+        x = np.exp(2j * np.pi * ph / N).astype(np.complex64)
+        for inv in (False, True):
+            y = c2c_dev(torch.from_numpy(x).cuda(), 3, inverse=inv).cpu().numpy()
+            k = ((-kz) % N, (-ky) % N, (-kx) % N) if inv else (kz, ky, kx)
+            assert np.unravel_index(np.argmax(np.abs(y)), y.shape) == k
+            peak = y[k]
+            y[k] = 0
+            print(f"tone {(kz, ky, kx)} inv={inv} peak={peak} max_rest={np.abs(y).max():.3e}")
+            assert abs(peak - N ** 3) <= 1e-4 * N ** 3
+            assert np.abs(y).max() <= 1e-3 * N ** 1.5  # the bar of the other index-map tests
+
+
+def test_cube_split_unaligned_input(gpu_lib, orc):
+    """The y-split pair on an input that is only 8-byte aligned."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    x = orc.uniform(1 << 24, 46)
+    base = torch.zeros((1 << 24) + 1, dtype=torch.complex64, device="cuda")
+    base[1:] = torch.from_numpy(x).cuda()
+    xt = base[1:].view(256, 256, 256)
+    assert xt.data_ptr() % 16 == 8
+    y = c2c_dev(xt, 3)
+    torch.cuda.synchronize()
+    assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [256, 256, 256])) <= TOL

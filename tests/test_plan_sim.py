@@ -126,6 +126,26 @@ def run_plan(passes, final_copy, x, inverse, aliased=False):
             bufs[p["dst"]][:] = X.reshape(-1)
             cur = p["dst"]
             continue
+        if p["mode"] in (4, 5):  # 256^3 blocks, y split 64 x 4 (y = a + 4 b, ky = k + 64 d)
+            def dft(v, ax):
+                return np.fft.ifft(v, axis=ax) * v.shape[ax] if inverse else np.fft.fft(v, axis=ax)
+            assert p["src"] == cur
+            if p["mode"] == 4:  # x, then over b; the column DFT of residue a to row 64 a + k
+                assert p["transposing"] and p["src"] != p["dst"]
+                assert p["in_o"] == 65536 and p["nouter"] * 65536 == total
+                pl = bufs[p["src"]].reshape(p["nouter"], 256, 256)
+                res = np.empty_like(pl)
+                for a in range(4):
+                    res[:, 64 * a:64 * a + 64, :] = dft(dft(pl[:, a::4, :], 2), 1)
+            else:  # row k + 64 a times W256^{a k}, over a to rows k + 64 d, then z
+                assert p["in_o"] == 1 << 24 and p["nouter"] << 24 == total
+                blocks = bufs[p["src"]].reshape(p["nouter"], 256, 4, 64, 256)
+                ak = np.arange(4)[:, None] * np.arange(64)[None, :]
+                w = np.exp((2j if inverse else -2j) * np.pi * ak / 256)
+                res = dft(dft(blocks * w[None, None, :, :, None], 2), 1)
+            bufs[p["dst"]][:] = res.reshape(-1)
+            cur = p["dst"]
+            continue
         ncb = (p["ncol"] + C - 1) // C
         # every active line (o, k, col); the grid covers col < ncb*C, inactive ones store nothing
         o, k, col = np.meshgrid(np.arange(p["nouter"]), np.arange(p["K"]), np.arange(ncb * C),
@@ -171,7 +191,7 @@ CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16]
          ([2, 1, 4], 1, 0), ([256], 5, 1), ([1 << 14], 2, 1), ([1 << 14], 2, 0), ([1 << 16], 1, 1),
          ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0), ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
          ([256, 256], 1, 4), ([2, 256, 256], 1, 4), ([4, 128, 128], 1, 0), ([128, 64], 3, 0),
-         ([64, 256, 256], 1, 0), ([256, 256], 65, 0),
+         ([64, 256, 256], 1, 0), ([256, 256], 65, 0), ([256, 256, 256], 1, 0), ([256, 256, 256], 1, 8),
          ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4), ([2, 512, 64], 1, 2),
          ([4, 128, 128], 1, 2), ([4096, 1024], 1, 0), ([4, 4096, 256], 1, 0), ([4096, 16], 64, 0)]
 
@@ -193,7 +213,8 @@ def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
 @pytest.mark.parametrize("dims,batch,flags", [([64, 64, 64], 1, 0), ([2048, 32], 1, 0),
                                               ([1 << 14], 1, 1), ([1 << 14], 1, 0),
                                               ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
-                                              ([64, 256, 256], 1, 0), ([4096, 1024], 1, 0)])
+                                              ([64, 256, 256], 1, 0), ([4096, 1024], 1, 0),
+                                              ([256, 256, 256], 1, 0)])
 def test_plan_replay_in_place(fl, orc, dims, batch, flags):
     passes, final_copy = describe(fl, dims, batch, flags, aliased=True)
     n = int(np.prod(dims)) * batch
@@ -205,12 +226,22 @@ def test_plan_replay_in_place(fl, orc, dims, batch, flags):
 
 
 def test_plan_shapes_of_headline_configs(fl):
-    # 256^3: the fused x+y plane pass, then the strided z pass, no temporary buffer
+    # 256^3: the y-split pair (x and y over b, then y over a and z), no temporary buffer
     p, fc = describe(fl, [256, 256, 256], 1)
-    assert [q["mode"] for q in p] == [2, 1] and not fc and all(q["dst"] == 1 for q in p)
-    # in place: the plane pass writes the temporary, the z pass writes back
+    assert [q["mode"] for q in p] == [4, 5] and not fc and all(q["dst"] == 1 for q in p)
+    assert [(q["src"], q["dst"]) for q in p] == [(0, 1), (1, 1)]
+    # in place: pass A writes the temporary, pass B writes back
     p, fc = describe(fl, [256, 256, 256], 1, aliased=True)
     assert [(q["src"], q["dst"]) for q in p] == [(1, 2), (2, 1)] and not fc
+    # batches of 256^3 blocks keep the pair
+    assert [q["mode"] for q in describe(fl, [256, 256, 256], 3)[0]] == [4, 5]
+    # without the y split: the fused x+y plane pass, then the strided z pass
+    p, fc = describe(fl, [256, 256, 256], 1, flags=8)
+    assert [q["mode"] for q in p] == [2, 1] and not fc and all(q["dst"] == 1 for q in p)
+    p, fc = describe(fl, [256, 256, 256], 1, flags=8, aliased=True)
+    assert [(q["src"], q["dst"]) for q in p] == [(1, 2), (2, 1)] and not fc
+    # other depths over 256 x 256 planes keep the plane pass
+    assert [q["mode"] for q in describe(fl, [128, 256, 256], 1)[0]] == [2, 1]
     # without plane fusion: three single passes (x contiguous, y and z strided)
     p, fc = describe(fl, [256, 256, 256], 1, flags=4)
     assert [q["mode"] for q in p] == [0, 1, 1] and not fc and all(q["dst"] == 1 for q in p)
@@ -251,7 +282,7 @@ def test_plan_bounds_of_large_arrays(fl, dims, batch):
     assert total * 8 >= 1 << 32
     for p in passes:
         L = 1 << p["lg"]
-        if p["mode"] == 2:
+        if p["mode"] in (2, 4, 5):
             assert p["nouter"] * p["in_o"] == total
             continue
         for s, o in (("in_p", "in_o"), ("out_p", "out_o")):
