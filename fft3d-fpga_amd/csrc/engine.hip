@@ -784,7 +784,16 @@ struct MultiPlan {
   bool inverse = false;
   std::string sig;
   size_t points = 0;
+  // mode 2 with several shape groups: one stream per group (default
+  // priority) forked from and joined back to the caller's stream, so the
+  // groups' launch chains overlap (also as parallel branches of a HIP graph)
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> aux_done;
+  hipEvent_t fork_ev = nullptr;
   ~MultiPlan() {
+    for (hipStream_t s : aux) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : aux_done) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (d_passes) (void)hipFree(d_passes);
     if (d_items) (void)hipFree(d_items);
     if (d_phase) (void)hipFree(d_phase);
@@ -967,6 +976,60 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
   return 0;
 }
 
+// the streams and events of the mode-2 fork (created outside any capture)
+static int multi_prepare_fork(MultiPlan* mp) {
+  if (mp->groups.size() < 2 || !mp->aux.empty()) return 0;
+  HIP_TRY(hipEventCreateWithFlags(&mp->fork_ev, hipEventDisableTiming));
+  for (size_t i = 0; i < mp->groups.size(); ++i) {
+    hipStream_t s = nullptr;
+    hipEvent_t e = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    mp->aux.push_back(s);
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    mp->aux_done.push_back(e);
+  }
+  return 0;
+}
+
+// x and y of one shape group as one plane-kernel launch, then its z axis as
+// one strided line pass over all its transforms, in place
+static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stream) {
+  PlaneArgs a{};
+  a.in = mp->din;
+  a.out = mp->dout;
+  a.nplanes = (unsigned long long)g.count * g.ppt;
+  a.plane_stride = (long long)g.ly * g.lx;
+  if (sched_np(g.lx) > 1 && get_stockham(PREC_F32, g.lx, &a.tw_x)) return -1;
+  if (sched_np(g.ly) > 1 && get_stockham(PREC_F32, g.ly, &a.tw_y)) return -1;
+  a.inverse = mp->inverse ? 1 : 0;
+  a.tab = mp->d_tab + 2 * g.tab_off;
+  a.ppt = g.ppt;
+  if (launch_fft_plane(g.ly, g.lx, a, stream, mp->split)) {
+    set_error("plane kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return -1;
+  }
+  const long long L = g.ppt, s = (long long)g.ly * g.lx;
+  PassArgs z{};
+  z.in = mp->dout;
+  z.out = mp->dout;
+  z.in_p = z.out_p = s;
+  z.in_c = z.out_c = 1;
+  z.ncol = (unsigned)s;
+  const int zc = geom_cols_for((int)L, MODE_STRIDED, mp->split && L >= 16);
+  z.ncb = (unsigned)((s + zc - 1) / zc);
+  z.K = 1;
+  z.inverse = mp->inverse ? 1 : 0;
+  z.tab = mp->d_tab_z + 2 * g.tab_off;
+  if (sched_np((int)L) > 1 && get_stockham(PREC_F32, (int)L, &z.tw)) return -1;
+  // split precision applies to lines with a radix-16 stage, as in the line passes
+  if (launch_fft_pass(PREC_F32, ilog2c(L), MODE_STRIDED, EPI_NONE, z, (unsigned long long)z.ncb * g.count,
+                      stream, mp->split && L >= 16)) {
+    set_error("z pass launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return -1;
+  }
+  return 0;
+}
+
 int multi_exec(void* handle, void* stream) {
   MultiPlan* mp = static_cast<MultiPlan*>(handle);
   if (!mp) {
@@ -974,50 +1037,27 @@ int multi_exec(void* handle, void* stream) {
     return -1;
   }
   if (mp->nphase == 0) return 0;
+  hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
   if (mp->mode == 2) {
     // no grid barrier in this schedule: clear what an earlier launch reported
     if (reset_multi_ctl(mp->d_ctl, stream)) {
       set_error("multi-transform reset failed: %s", hipGetErrorString(hipGetLastError()));
       return -1;
     }
-    for (const PlaneGroup& g : mp->groups) {
-      PlaneArgs a{};
-      a.in = mp->din;
-      a.out = mp->dout;
-      a.nplanes = (unsigned long long)g.count * g.ppt;
-      a.plane_stride = (long long)g.ly * g.lx;
-      if (sched_np(g.lx) > 1 && get_stockham(PREC_F32, g.lx, &a.tw_x)) return -1;
-      if (sched_np(g.ly) > 1 && get_stockham(PREC_F32, g.ly, &a.tw_y)) return -1;
-      a.inverse = mp->inverse ? 1 : 0;
-      a.tab = mp->d_tab + 2 * g.tab_off;
-      a.ppt = g.ppt;
-      if (launch_fft_plane(g.ly, g.lx, a, stream, mp->split)) {
-        set_error("plane kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-        return -1;
-      }
+    if (mp->groups.size() < 2) {
+      for (const PlaneGroup& g : mp->groups)
+        if (multi_group_exec(mp, g, hs)) return -1;
+      return 0;
     }
-    // z axis of each group: one strided line pass over all its transforms, in place
-    for (const PlaneGroup& g : mp->groups) {
-      const long long L = g.ppt, s = (long long)g.ly * g.lx;
-      PassArgs z{};
-      z.in = mp->dout;
-      z.out = mp->dout;
-      z.in_p = z.out_p = s;
-      z.in_c = z.out_c = 1;
-      z.ncol = (unsigned)s;
-      const int zc = geom_cols_for((int)L, MODE_STRIDED, mp->split && L >= 16);
-      z.ncb = (unsigned)((s + zc - 1) / zc);
-      z.K = 1;
-      z.inverse = mp->inverse ? 1 : 0;
-      z.tab = mp->d_tab_z + 2 * g.tab_off;
-      if (sched_np((int)L) > 1 && get_stockham(PREC_F32, (int)L, &z.tw)) return -1;
-      // split precision applies to lines with a radix-16 stage, as in the line passes
-      if (launch_fft_pass(PREC_F32, ilog2c(L), MODE_STRIDED, EPI_NONE, z,
-                          (unsigned long long)z.ncb * g.count, stream, mp->split && L >= 16)) {
-        set_error("z pass launch failed: %s", hipGetErrorString(hipGetLastError()));
-        return -1;
-      }
+    // the groups touch disjoint transforms: each chain on its own stream
+    if (multi_prepare_fork(mp)) return -1;
+    HIP_TRY(hipEventRecord(mp->fork_ev, hs));
+    for (size_t i = 0; i < mp->groups.size(); ++i) {
+      HIP_TRY(hipStreamWaitEvent(mp->aux[i], mp->fork_ev, 0));
+      if (multi_group_exec(mp, mp->groups[i], mp->aux[i])) return -1;
+      HIP_TRY(hipEventRecord(mp->aux_done[i], mp->aux[i]));
     }
+    for (size_t i = 0; i < mp->groups.size(); ++i) HIP_TRY(hipStreamWaitEvent(hs, mp->aux_done[i], 0));
     return 0;
   }
   if (mp->mode == 0) {
@@ -1182,6 +1222,13 @@ int graph_capture_multi(void** out, void** handles, int n, void* stream) {
     return -1;
   }
   auto gr = std::make_unique<Graph>();
+  for (int i = 0; i < n; ++i) {
+    if (!handles[i]) {
+      set_error("null multi-transform handle");
+      return -1;
+    }
+    if (multi_prepare_fork(static_cast<MultiPlan*>(handles[i]))) return -1;  // no creation under capture
+  }
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   for (int i = 0; i < n && !rc; ++i) rc = multi_exec(handles[i], s);
