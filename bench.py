@@ -23,7 +23,8 @@ steps with 256 x 256 planes as separate x and y passes), torch_fft_rocfft
 grouped-plane launch against a per-transform loop), split_512 (split-precision
 512^3), autograd_16x128 (differentiable forward/backward), host_api_pcie (the
 drop-in call on host buffers: copy-in, transform, copy-out times), cpu_fftwf
-(threaded FFTW API on this host).
+(threaded FFTW API on this host), other_workloads (64^3, 4096^2 and 2^22-point
+1D forward transforms).
 """
 from __future__ import annotations
 
@@ -84,6 +85,8 @@ def parse():
                     help="skip the differentiable 16 x 128^3 forward/backward measurement")
     ap.add_argument("--no-cpu-fftw", action="store_true",
                     help="skip the threaded FFTW-API CPU reference (MKL's fftwf wrappers)")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the other single-transform configurations (64^3, 4096^2, 2^22 1D)")
     ap.add_argument("--no-host-api", action="store_true",
                     help="skip the drop-in host-buffer call (PCIe-inclusive timing)")
     return ap.parse_args()
@@ -260,6 +263,57 @@ def autograd_16x128(steps):
             "roofline_frac_backward": round(alg_b / (ms_b * 1e-3) / HBM_PEAK, 4),
             "timing": "events on the current stream around the forward call and around backward(), "
                       "steps issued back to back"}
+
+
+def other_workloads(steps, warmup):
+    """The other single-transform configurations, forward, HBM resident, through
+    the device entry point (64^3 seed 0; 4096^2 and 2^22-point 1D with
+    bit-reversed output, seed 1): wall-clock ms per transform over `steps`
+    back-to-back calls, and each kernel's mean launch time from HIP events in a
+    second pass over the same steps."""
+    import torch
+
+    import fftfpga_amd as fl
+    from fftfpga_amd import datagen
+    lib = fl.load()
+    res = {}
+    for name in ("3d_64_fwd", "2d_4096_fwd", "1d_4m_fwd"):
+        dims, batch, seed, _ = WORKLOADS[name]
+        rank = len(dims)
+        npts = math.prod(dims) * batch
+        x = torch.from_numpy(datagen.uniform(npts, seed)).cuda().reshape(batch, *dims)
+        X = torch.empty_like(x)
+        bitrev = rank == 1
+
+        def step():
+            fl.c2c_dev(x, rank, out=X, bitrev_1d=bitrev)
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        if lib.fftamd_profile_begin(steps * 4 + 16):
+            raise RuntimeError(fl.last_error())
+        for _ in range(steps):
+            step()
+        buf = ctypes.create_string_buffer(1 << 16)
+        if lib.fftamd_profile_end(buf, len(buf)) < 0:
+            raise RuntimeError(fl.last_error())
+        kern = json.loads(buf.value.decode())
+        gbs = 16.0 * npts / (ms * 1e-3)
+        res[name] = {"dims": dims, "ms_per_transform": round(ms, 5),
+                     "gflops": round(flops_per_transform(dims) * batch / (ms * 1e-3) / 1e9, 1),
+                     "hbm_gbs_algorithmic": round(gbs / 1e9, 1),
+                     "roofline_frac": round(gbs / HBM_PEAK, 4),
+                     "kernels_us": {k: round(1e3 * v["total_ms"] / v["count"], 3) for k, v in kern.items()},
+                     "output_order": "bit-reversed per transform" if bitrev else "natural"}
+        del x, X
+    torch.cuda.empty_cache()
+    return res
 
 
 def host_api_pcie(N=256, seed=1, reps=3):
@@ -545,6 +599,8 @@ def main():
         result["split_512"] = split_512(max(3, min(args.steps, 10)))
     if not args.no_autograd:
         result["autograd_16x128"] = autograd_16x128(max(3, min(args.steps, 10)))
+    if args.workload == "3d_256_fwd_inv" and not args.no_other:
+        result["other_workloads"] = other_workloads(max(5, min(args.steps, 50)), args.warmup)
     if not args.no_host_api:
         result["host_api_pcie"] = host_api_pcie()
     if not args.no_cpu_baseline:
