@@ -189,6 +189,65 @@ def test_graph_forward_inverse_pair_replays(gpu_lib):
     g.close()
 
 
+def test_graph_plane_schedule_pair_replays(gpu_lib):
+    """Mode 2 with three shape groups runs each group's launches on its own
+    forked stream: captured into a graph they are parallel branches.  A
+    forward/inverse pair replayed from the graph equals the eager pair on
+    every replay, with eager work between the replays."""
+    import torch
+    from fftfpga_amd.multi import Graph, MultiPlan
+    shapes = [(32, 32, 32)] * 6 + [(64, 64, 64)] * 3 + [(128, 128, 128)] * 2
+    host, d_in, offs = _pack(shapes, 10)
+    mid = torch.empty_like(d_in)
+    back = torch.empty_like(d_in)
+    fwd = MultiPlan(shapes, d_in, mid)
+    inv = MultiPlan(shapes, mid, back, inverse=True)
+    for p in (fwd, inv):
+        sch = p.schedule()
+        p.set_schedule(2, sch["grid"], sch["chunk"])
+    fwd()
+    inv()
+    inv.check()
+    eager_mid, eager = mid.clone(), back.clone()
+    g = Graph([fwd, inv])
+    for r in range(3):
+        back.zero_()
+        mid.mul_(0)
+        d_in.add_(0)  # eager work between replays
+        torch.cuda.synchronize()
+        g()
+        torch.cuda.synchronize()
+        assert torch.equal(mid, eager_mid), f"replay {r}: forward differs from the eager result"
+        assert torch.equal(back, eager), f"replay {r}: inverse differs from the eager result"
+    out = back.cpu().numpy()
+    for s, o in zip(shapes, offs):
+        n = int(np.prod(s))
+        err = np.linalg.norm(out[o:o + n] / n - host[o:o + n]) / np.linalg.norm(host[o:o + n])
+        assert err <= TOL
+    g.close()
+    fwd.close()
+    inv.close()
+
+
+def test_plane_schedule_single_group(gpu_lib, orc):
+    """Mode 2 with one shape group (no stream fork) against the oracle."""
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    shapes = [(64, 64, 64)] * 3
+    host, d_in, offs = _pack(shapes, 11)
+    d_out = torch.empty_like(d_in)
+    plan = MultiPlan(shapes, d_in, d_out)
+    sch = plan.schedule()
+    plan.set_schedule(2, sch["grid"], sch["chunk"])
+    plan()
+    plan.check()
+    out = d_out.cpu().numpy()
+    for s, o in zip(shapes, offs):
+        n = int(np.prod(s))
+        assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s))) <= TOL
+    plan.close()
+
+
 def test_barrier_bound_is_reported_not_hung(gpu_lib):
     import torch
     from fftfpga_amd.multi import MultiPlan, set_spin_limit
