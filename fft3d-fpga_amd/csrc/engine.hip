@@ -312,33 +312,34 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   const bool bitrev = (flags & FLAG_BITREV_1D) && rank == 1;
   std::vector<PassDesc> ps;
 
-  // 256 x 256 x 256 blocks (the three fastest axes): two passes with the y
-  // axis split 64 x 4 between them -- x and y over b (out of place), then y
-  // over a and z (in place)
+  // n x n x n blocks, n = 256 or 128 (the three fastest axes): two passes with
+  // the y axis split (n/4) x 4 between them -- x and y over b (out of place),
+  // then y over a and z (in place)
   int top = rank - 1;
+  const long long CN = rank >= 3 ? (long long)dims[rank - 1] : 0;
   if (rank >= 3 && prec == PREC_F32 && !plan.split && !(flags & (FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT)) &&
-      dims[rank - 1] == 256 && dims[rank - 2] == 256 && dims[rank - 3] == 256) {
+      (CN == 256 || CN == 128) && (long long)dims[rank - 2] == CN && (long long)dims[rank - 3] == CN) {
     unsigned long long O = batch;
     for (int d = 0; d < rank - 3; ++d) O *= dims[d];
     PassDesc A;
-    A.lg = 8;
+    A.lg = ilog2c(CN);
     A.mode = MODE_CUBE_A;
     A.transposing = true;  // residue a's output rows are other residues' input rows
     A.in_p = A.out_p = 1;
-    A.in_c = A.out_c = 256;
-    A.ncol = 256;
-    A.K = 256;
-    A.nouter = O * 256;  // planes
-    A.in_o = A.out_o = 256 * 256;
+    A.in_c = A.out_c = CN;
+    A.ncol = (unsigned)CN;
+    A.K = (unsigned)CN;
+    A.nouter = O * (unsigned long long)CN;  // planes
+    A.in_o = A.out_o = CN * CN;
     ps.push_back(A);
     PassDesc B;
-    B.lg = 8;
+    B.lg = ilog2c(CN);
     B.mode = MODE_CUBE_B;
-    B.in_p = B.out_p = 256 * 256;
+    B.in_p = B.out_p = CN * CN;
     B.in_c = B.out_c = 1;
-    B.ncol = 256 * 256;
+    B.ncol = (unsigned)(CN * CN);
     B.nouter = O;  // blocks
-    B.in_o = B.out_o = 256LL * 256 * 256;
+    B.in_o = B.out_o = CN * CN * CN;
     ps.push_back(B);
     top = rank - 4;
   }
@@ -623,8 +624,9 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       CubeArgs ca{};
       ca.in = ptr(p.src);
       ca.out = ptr(p.dst);
-      ca.nplanes = A ? p.nouter : p.nouter * 256;
-      if (get_stockham(PREC_F32, 256, &ca.tw256)) return -1;
+      ca.n = 1 << p.lg;
+      ca.nplanes = A ? p.nouter : p.nouter * (unsigned long long)ca.n;
+      if (get_stockham(PREC_F32, ca.n, &ca.tw)) return -1;
       if (get_direct(256, &ca.w256)) return -1;
       // pass A is always the first pass of its plan; pass B takes the store
       // weights when it is the last
@@ -638,11 +640,13 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       const int rc = A ? launch_fft_cube_a(ca, stream) : launch_fft_cube_b(ca, stream);
       if (prof) {
         HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
-        pf.recs.push_back({A ? "fft_cube_a_f32_256" : "fft_cube_b_f32_256", pf.pool[pf.used], pf.pool[pf.used + 1]});
+        char lab[64];
+        snprintf(lab, sizeof(lab), "fft_cube_%s_f32_%d", A ? "a" : "b", ca.n);
+        pf.recs.push_back({lab, pf.pool[pf.used], pf.pool[pf.used + 1]});
         pf.used += 2;
       }
       if (rc) {
-        set_error("256^3 y-split pass launch failed (rc=%d): %s", rc, hipGetErrorString(hipGetLastError()));
+        set_error("y-split pass launch failed (n=%d, rc=%d): %s", ca.n, rc, hipGetErrorString(hipGetLastError()));
         return -1;
       }
       cur = p.dst;
@@ -1432,11 +1436,12 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
                        (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
                        d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K)
                        : d.mode == MODE_COL4096 ? 16
-                       : d.mode == MODE_CUBE_A ? 64   // rows per workgroup
-                       : d.mode == MODE_CUBE_B ? 16   // columns per workgroup (x 4 rows x 256 planes)
+                       : d.mode == MODE_CUBE_A ? (1LL << d.lg) / 4   // rows per workgroup
+                       : d.mode == MODE_CUBE_B ? 16   // columns per workgroup (x 4 rows x n planes)
                        : geom_cols_for(1 << d.lg, d.mode, sp(d)),
                        d.mode == MODE_PLANE ? plane_threads(d)
-                       : d.mode == MODE_COL4096 || d.mode == MODE_CUBE_A || d.mode == MODE_CUBE_B ? 512
+                       : d.mode == MODE_COL4096 ? 512
+                       : d.mode == MODE_CUBE_A || d.mode == MODE_CUBE_B ? (d.lg == 8 ? 512 : 256)
                        : geom_threads_for(1 << d.lg, d.mode, sp(d))};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }

@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
   const long long base = (long long)blk * CB_BLOCK + (long long)k * CB_L + 16 * xg;
   const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
   cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
-  const cpx<float>* __restrict__ tw256 = reinterpret_cast<const cpx<float>*>(a.tw256);
+  const cpx<float>* __restrict__ tw256 = reinterpret_cast<const cpx<float>*>(a.tw);
   const int tid = threadIdx.x;
 
   // ---- y over a: thread (x, zs) holds planes z = zs + 32 j; t[j][d] = X[k + 64 d].
@@ -194,9 +194,142 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
   }
 }
 
+// ---- 128 x 128 x 128 blocks: the same split, y = a + 4b (b < 32), ky = k + 32d.
+// Smaller tiles: pass A holds 32 rows of 128 (32 KiB) in LDS with 256 threads
+// of 16 points; pass B 16 columns x 4 rows x 128 planes (64 KiB) with 256
+// threads of 32 points.  Four workgroups per CU.
+constexpr int C1_L = 128;
+constexpr int C1_NT = 256;
+constexpr long long C1_PLANE = (long long)C1_L * C1_L;
+constexpr long long C1_BLOCK = C1_PLANE * C1_L;
+using C1Row = Geom<C1_L, MODE_CONTIG, 32>;   // pass A rows: 32 rows of 128, 8 threads each
+using C1Col = Geom<32, MODE_STRIDED, 128>;   // pass A columns: 128 columns of 32, 2 threads each
+using C1Z = Geom<C1_L, MODE_STRIDED, 32>;    // pass B: 32 lines (2 rows x 16 columns) of 128
+static_assert(C1Row::NT == C1_NT && C1Col::NT == C1_NT && C1Z::NT == C1_NT, "256 threads");
+static_assert(sched_np(C1_L) == 2 && sched_np(32) == 2, "two stages per line");
+constexpr int C1A_LDS = C1Row::LDS_ELEMS > C1Col::LDS_ELEMS ? C1Row::LDS_ELEMS : C1Col::LDS_ELEMS;
+
+template <bool INV, bool WL>
+__global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
+  __shared__ cpx<float> buf[C1A_LDS];
+  const unsigned long long plane = blockIdx.x >> 2;
+  const int res = blockIdx.x & 3;  // residue class a of the input rows
+  if (plane >= a.nplanes) return;  // uniform over the workgroup
+  const long long base = (long long)plane * C1_PLANE;
+  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
+  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
+  const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
+  const int tid = threadIdx.x;
+
+  // ---- x: thread (c, tau) transforms row b = c (y = res + 4c); its stage-0
+  // inputs come straight from HBM; it ends with point kx = tau + 8 r
+  const int c = tid >> 3, tau = tid & 7;
+  cpx<float> v[16];
+  {
+    constexpr int RAD = sched_radix(C1_L, 0), NB = 16 / RAD;
+    const long long row = (long long)(res + 4 * c) * C1_L;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        const long long e = row + tau + C1Row::TPL * bb + r * (C1_L / RAD);
+        cpx<float> s = in[e];
+        if (WL) s = cscale(s, a.w_load[(unsigned long long)(base + e) & a.w_mask]);
+        v[bb * RAD + r] = INV ? cswap(s) : s;
+      }
+  }
+  static_assert(sched_ns(C1_L, 1) == C1Row::TPL, "last row stage: point tau + 8 r");
+  run_stages<float, C1_L, MODE_CONTIG, 0, false, 32>(v, buf, c, tau, tw);
+
+  // ---- y over b: 128 columns of 32 points; thread (cc, tq) ends with point
+  // k = tq + 2 r of its column
+  __syncthreads();  // every thread has read its last row-stage inputs
+#pragma unroll
+  for (int r = 0; r < 16; ++r) buf[C1Col::idx(tau + 8 * r, c)] = v[r];
+  __syncthreads();
+  const int cc = tid & 127;
+  const int tq = __builtin_amdgcn_readfirstlane(tid >> 7);  // uniform over each wave
+  stage_load_lds<float, 32, MODE_STRIDED, 0, 128>(v, buf, cc, tq);
+  __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
+  stage_compute<float, 32, MODE_STRIDED, 0, false, 128>(v, tq, nullptr);  // eight radix-2, no twiddles
+  stage_store_lds<float, 32, MODE_STRIDED, 0, 128>(v, buf, cc, tq);
+  __syncthreads();
+  stage_load_lds<float, 32, MODE_STRIDED, 1, 128>(v, buf, cc, tq);
+  // radix-16 stage over sub-transforms of span 2: twiddles W32^{tq r} = W256^{8 tq r}
+#pragma unroll
+  for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[8 * tq * r]);
+  dft16(v);
+  // column DFT point k = tq + 2 r to row 32 res + k (pass B applies W128^{res k})
+#pragma unroll
+  for (int r = 0; r < 16; ++r) out[(long long)(32 * res + tq + 2 * r) * C1_L + cc] = v[r];
+}
+
+template <bool INV, bool WS>
+__global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_b(const CubeArgs a) {
+  __shared__ cpx<float> buf[C1Z::LDS_ELEMS];
+  const unsigned b = blockIdx.x;
+  const int xg = b & 7;                // group of 16 columns
+  const int k = (b >> 3) & 31;         // rows k + 32 a
+  const unsigned long long blk = b >> 8;
+  if (blk >= a.nplanes / C1_L) return;  // uniform over the workgroup
+  const long long base = (long long)blk * C1_BLOCK + (long long)k * C1_L + 16 * xg;
+  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
+  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
+  const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
+  const int tid = threadIdx.x;
+
+  // ---- y over a: thread (x, zs) holds planes z = zs + 16 j; t[j][d] = X[k + 32 d]
+  const int x = tid & 15, zs = tid >> 4;
+  cpx<float> t[8][4];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const long long zo = (long long)(zs + 16 * j) * C1_PLANE + x;
+#pragma unroll
+    for (int aa = 0; aa < 4; ++aa) t[j][aa] = in[zo + 32 * C1_L * aa];
+  }
+  // T_a[k] = W128^{a k} = W256^{2 a k} times pass A's column DFT (uniform)
+  const cpx<float> w1 = wdir[2 * k], w2 = wdir[4 * k], w3 = wdir[6 * k];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    t[j][1] = cmul(t[j][1], w1);
+    t[j][2] = cmul(t[j][2], w2);
+    t[j][3] = cmul(t[j][3], w3);
+    dft4(t[j]);
+  }
+
+  // ---- z: the 32 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 16 (d - 2 e) + x
+  const int cz = tid & 31;
+  const int tz = tid >> 5;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    if (e) __syncthreads();  // the first half's last stage has read the buffer
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      buf[C1Z::idx(x, zs + 16 * j)] = t[j][2 * e];
+      buf[C1Z::idx(16 + x, zs + 16 * j)] = t[j][2 * e + 1];
+    }
+    __syncthreads();
+    cpx<float> v[16];
+    stage_load_lds<float, C1_L, MODE_STRIDED, 0, 32>(v, buf, cz, tz);
+    __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
+    run_stages<float, C1_L, MODE_STRIDED, 0, false, 32>(v, buf, cz, tz, tw);
+    // point kz = tz + 8 r of row k + 32 d, column 16 xg + (cz & 15)
+    const long long ob = (long long)(32 * (2 * e + (cz >> 4))) * C1_L + (cz & 15);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const long long o = ob + (long long)(tz + 8 * r) * C1_PLANE;
+      cpx<float> s = INV ? cswap(v[r]) : v[r];
+      if (WS) s = cscale(s, a.w_store[(unsigned long long)(base + o) & a.w_mask]);
+      __builtin_nontemporal_store(__builtin_bit_cast(long long, s), reinterpret_cast<long long*>(out + o));
+    }
+  }
+}
+
 static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsigned long long* blocks) {
   const unsigned long long b = nplanes * per;
-  if (b > 0xFFFFFFFFull / CB_NT) return false;  // grid size in work-items is 32-bit
+  if (b > 0xFFFFFFFFull / CB_NT) return false;  // (512 threads bounds both kernels' grids)  // grid size in work-items is 32-bit
   *blocks = b;
   return true;
 }
@@ -204,7 +337,19 @@ static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsign
 int launch_fft_cube_a(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
-  if (a.nplanes == 0 || !cube_grid(a.nplanes, 4, &blocks) || a.w_store) return -1;
+  if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || !cube_grid(a.nplanes, 4, &blocks) || a.w_store)
+    return -1;
+  if (a.n == C1_L) {
+    const dim3 g((unsigned)blocks), t(C1_NT);
+    if (a.inverse) {
+      if (a.w_load) hipLaunchKernelGGL((k_fft_cube128_a<true, true>), g, t, 0, s, a);
+      else hipLaunchKernelGGL((k_fft_cube128_a<true, false>), g, t, 0, s, a);
+    } else {
+      if (a.w_load) hipLaunchKernelGGL((k_fft_cube128_a<false, true>), g, t, 0, s, a);
+      else hipLaunchKernelGGL((k_fft_cube128_a<false, false>), g, t, 0, s, a);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
   const dim3 g((unsigned)blocks), t(CB_NT);
   if (a.inverse) {
     if (a.w_load) hipLaunchKernelGGL((k_fft_cube_a<true, true>), g, t, 0, s, a);
@@ -219,8 +364,20 @@ int launch_fft_cube_a(const CubeArgs& a, void* stream) {
 int launch_fft_cube_b(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
-  if (a.nplanes == 0 || a.nplanes % CB_L || !cube_grid(a.nplanes / CB_L, 1024, &blocks) || a.w_load)
-    return -1;
+  if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || a.nplanes % a.n || a.w_load) return -1;
+  if (a.n == C1_L) {
+    if (!cube_grid(a.nplanes / C1_L, 256, &blocks)) return -1;
+    const dim3 g((unsigned)blocks), t(C1_NT);
+    if (a.inverse) {
+      if (a.w_store) hipLaunchKernelGGL((k_fft_cube128_b<true, true>), g, t, 0, s, a);
+      else hipLaunchKernelGGL((k_fft_cube128_b<true, false>), g, t, 0, s, a);
+    } else {
+      if (a.w_store) hipLaunchKernelGGL((k_fft_cube128_b<false, true>), g, t, 0, s, a);
+      else hipLaunchKernelGGL((k_fft_cube128_b<false, false>), g, t, 0, s, a);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
+  if (!cube_grid(a.nplanes / CB_L, 1024, &blocks)) return -1;
   const dim3 g((unsigned)blocks), t(CB_NT);
   if (a.inverse) {
     if (a.w_store) hipLaunchKernelGGL((k_fft_cube_b<true, true>), g, t, 0, s, a);

@@ -60,17 +60,18 @@ struct ColArgs {
 int launch_fft_col4096(const ColArgs& a, void* stream);
 bool col4096_supported(long long lx, unsigned long long nouter);
 
-// Arrays of 256 x 256 x 256 blocks in two passes with the y axis split 64 x 4
-// (fft_cube.hip).  Pass A: x and the 64-point y stage of every plane, out of
-// place only.  Pass B: the 4-point y stage and z of every block, in place
+// Arrays of n x n x n blocks (n = 256 or 128) in two passes with the y axis
+// split (n/4) x 4 (fft_cube.hip).  Pass A: x and the (n/4)-point y stage of
+// every plane, out of place only.  Pass B: the 4-point y stage and z of every block, in place
 // allowed.  Pass A takes load weights only, pass B store weights only (see
 // PassArgs).  0 on success, -1 for an unsupported shape or grid, -2 on a
 // launch error.
 struct CubeArgs {
   const void* in;
   void* out;
-  unsigned long long nplanes;    // 256 x 256 planes in the array (a multiple of 256 for pass B)
-  const void* tw256;             // Stockham twiddles of 256-point lines
+  int n;                         // block side: 256 or 128
+  unsigned long long nplanes;    // n x n planes in the array (a multiple of n for pass B)
+  const void* tw;                // Stockham twiddles of n-point lines
   const void* w256;              // W256^m, m = 0..255
   const float* w_load;
   const float* w_store;

@@ -36,11 +36,12 @@ extern "C" {
  * up) runs in one pass instead of a four-step pair.  This flag asks for the
  * separate line passes instead (it also turns off the y-split pair below). */
 #define FFTAMD_NO_PLANE_FUSION 0x4u
-/* Single-precision arrays whose three fastest axes are 256 x 256 x 256 run in
- * two passes with the y axis split 64 x 4 between them (x and 64-point y over
- * one residue class of rows per workgroup; then the 4-point y stage and z),
+/* Single-precision arrays whose three fastest axes are 256 x 256 x 256 or
+ * 128 x 128 x 128 run in two passes with the y axis split (n/4) x 4 between
+ * them (x and (n/4)-point y over one residue class of rows per workgroup; then
+ * the 4-point y stage and z),
  * so that every tile fits one workgroup and no input is read twice.  This
- * flag keeps the 256 x 256 plane pass and a separate z pass instead. */
+ * flag keeps the plane pass and a separate z pass instead. */
 #define FFTAMD_NO_CUBE_SPLIT 0x8u
 
 /* Single precision, rank 1..3, every dimension a power of two, `batch`

@@ -147,7 +147,8 @@ def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode):
     ref = torch.empty_like(d_in)
     for s, o in zip(shapes, offs):
         n = int(np.prod(s))
-        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s), plane_fusion=(mode == 2))
+        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s), plane_fusion=(mode == 2),
+                cube_split=False)  # mode 2 runs plane kernels, also for the 128^3 members
     torch.cuda.synchronize()
     assert torch.equal(d_out, ref)
     out = d_out.cpu().numpy()

@@ -373,7 +373,7 @@ def test_plane_pass_vs_oracle(gpu_lib, orc, shape, rank):
 
 @pytest.mark.parametrize("shape,rank", [((64, 256, 256), 2), ((64, 256, 256), 3), ((2, 128, 128), 2),
                                         ((4, 64, 64), 3), ((4096, 1024), 2), ((4, 4096, 256), 3),
-                                        ((256, 256, 256), 3), ((2, 256, 256, 256), 3)])
+                                        ((256, 256, 256), 3), ((2, 256, 256, 256), 3), ((2, 128, 128, 128), 3)])
 def test_plane_pass_weights(gpu_lib, orc, shape, rank):
     """Weights applied in the fused pass's loads (both ranks) and stores (rank 2);
     256^3 blocks: load weights in the y-split pair's first pass, store weights
@@ -430,19 +430,21 @@ def test_plane_pass_unaligned_input(gpu_lib, orc):
 
 # ---- 256^3 blocks: two passes with the y axis split 64 x 4 -----------------------------------------
 
-@pytest.mark.parametrize("shape", [(256, 256, 256), (2, 256, 256, 256)])
+@pytest.mark.parametrize("shape", [(256, 256, 256), (2, 256, 256, 256), (128, 128, 128), (3, 128, 128, 128)])
 def test_cube_split_vs_oracle(gpu_lib, orc, shape):
-    """The y-split pair (x and 64-point y over rows y = a + 4b; then the 4-point
-    y stage and z) against the FP64 oracle and against the plane-pass path,
-    out of place and in place, both directions, one block and a batch of two."""
+    """The y-split pair (x and (N/4)-point y over rows y = a + 4b; then the
+    4-point y stage and z) for N = 256 and 128 against the FP64 oracle and
+    against the plane-pass path, out of place and in place, both directions,
+    one block and a batch."""
     import torch
     from fftfpga_amd import c2c_dev
     n = int(np.prod(shape))
-    x = orc.uniform(n, 4100 + len(shape))
-    batch = n // (1 << 24)
+    N = shape[-1]
+    x = orc.uniform(n, 4100 + len(shape) + N)
+    batch = n // N ** 3
     xt = torch.from_numpy(x.reshape(shape)).cuda()
     for inv in (False, True):
-        ref = orc.fftnd(x, [256, 256, 256], batch=batch, inverse=inv)
+        ref = orc.fftnd(x, [N, N, N], batch=batch, inverse=inv)
         y = c2c_dev(xt, 3, inverse=inv)
         yp = c2c_dev(xt, 3, inverse=inv, cube_split=False)
         z = xt.clone()
@@ -456,14 +458,16 @@ def test_cube_split_vs_oracle(gpu_lib, orc, shape):
     assert np.array_equal(xt.cpu().numpy().reshape(-1), x)  # input untouched
 
 
-def test_cube_split_single_tones_exact(gpu_lib):
+@pytest.mark.parametrize("N", [256, 128])
+def test_cube_split_single_tones_exact(gpu_lib, N):
     """Single complex exponentials land exactly on their bins through the
     y-split pair (every y residue and output row block, both directions)."""
     import torch
     from fftfpga_amd import c2c_dev
-    N = 256
     i = np.arange(N, dtype=np.int64)
-    for (kz, ky, kx) in [(0, 0, 0), (1, 2, 3), (255, 64, 128), (17, 191, 5), (128, 255, 254)]:
+    q = N // 4
+    tones = [(0, 0, 0), (1, 2, 3), (N - 1, q, N // 2), (17, 3 * q - 1, 5), (N // 2, N - 1, N - 2)]
+    for (kz, ky, kx) in tones:
         ph = (kz * i[:, None, None] + ky * i[None, :, None] + kx * i[None, None, :]) % N
         x = np.exp(2j * np.pi * ph / N).astype(np.complex64)
         for inv in (False, True):

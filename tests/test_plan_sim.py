@@ -126,22 +126,25 @@ def run_plan(passes, final_copy, x, inverse, aliased=False):
             bufs[p["dst"]][:] = X.reshape(-1)
             cur = p["dst"]
             continue
-        if p["mode"] in (4, 5):  # 256^3 blocks, y split 64 x 4 (y = a + 4 b, ky = k + 64 d)
+        if p["mode"] in (4, 5):  # n^3 blocks, y split (n/4) x 4 (y = a + 4 b, ky = k + (n/4) d)
             def dft(v, ax):
                 return np.fft.ifft(v, axis=ax) * v.shape[ax] if inverse else np.fft.fft(v, axis=ax)
             assert p["src"] == cur
-            if p["mode"] == 4:  # x, then over b; the column DFT of residue a to row 64 a + k
+            n = 1 << p["lg"]
+            q = n // 4
+            assert n in (128, 256)
+            if p["mode"] == 4:  # x, then over b; the column DFT of residue a to row q a + k
                 assert p["transposing"] and p["src"] != p["dst"]
-                assert p["in_o"] == 65536 and p["nouter"] * 65536 == total
-                pl = bufs[p["src"]].reshape(p["nouter"], 256, 256)
+                assert p["in_o"] == n * n and p["nouter"] * n * n == total
+                pl = bufs[p["src"]].reshape(p["nouter"], n, n)
                 res = np.empty_like(pl)
                 for a in range(4):
-                    res[:, 64 * a:64 * a + 64, :] = dft(dft(pl[:, a::4, :], 2), 1)
-            else:  # row k + 64 a times W256^{a k}, over a to rows k + 64 d, then z
-                assert p["in_o"] == 1 << 24 and p["nouter"] << 24 == total
-                blocks = bufs[p["src"]].reshape(p["nouter"], 256, 4, 64, 256)
-                ak = np.arange(4)[:, None] * np.arange(64)[None, :]
-                w = np.exp((2j if inverse else -2j) * np.pi * ak / 256)
+                    res[:, q * a:q * a + q, :] = dft(dft(pl[:, a::4, :], 2), 1)
+            else:  # row k + q a times Wn^{a k}, over a to rows k + q d, then z
+                assert p["in_o"] == n ** 3 and p["nouter"] * n ** 3 == total
+                blocks = bufs[p["src"]].reshape(p["nouter"], n, 4, q, n)
+                ak = np.arange(4)[:, None] * np.arange(q)[None, :]
+                w = np.exp((2j if inverse else -2j) * np.pi * ak / n)
                 res = dft(dft(blocks * w[None, None, :, :, None], 2), 1)
             bufs[p["dst"]][:] = res.reshape(-1)
             cur = p["dst"]
@@ -192,6 +195,7 @@ CASES = [([8, 8, 8], 1, 0), ([64, 64, 64], 1, 0), ([4, 16, 32], 2, 0), ([16, 16]
          ([4096, 4], 1, 0), ([1 << 10, 1 << 10], 1, 0), ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
          ([256, 256], 1, 4), ([2, 256, 256], 1, 4), ([4, 128, 128], 1, 0), ([128, 64], 3, 0),
          ([64, 256, 256], 1, 0), ([256, 256], 65, 0), ([256, 256, 256], 1, 0), ([256, 256, 256], 1, 8),
+         ([128, 128, 128], 2, 0), ([128, 128, 128], 1, 8),
          ([2, 16, 64], 2, 0), ([32, 32, 32], 1, 0), ([64, 64, 64], 1, 4), ([2, 512, 64], 1, 2),
          ([4, 128, 128], 1, 2), ([4096, 1024], 1, 0), ([4, 4096, 256], 1, 0), ([4096, 16], 64, 0)]
 
@@ -214,7 +218,7 @@ def test_plan_replay_matches_oracle(fl, orc, dims, batch, flags, inverse):
                                               ([1 << 14], 1, 1), ([1 << 14], 1, 0),
                                               ([2, 256, 256], 1, 0), ([256, 256], 2, 0),
                                               ([64, 256, 256], 1, 0), ([4096, 1024], 1, 0),
-                                              ([256, 256, 256], 1, 0)])
+                                              ([256, 256, 256], 1, 0), ([128, 128, 128], 2, 0)])
 def test_plan_replay_in_place(fl, orc, dims, batch, flags):
     passes, final_copy = describe(fl, dims, batch, flags, aliased=True)
     n = int(np.prod(dims)) * batch
@@ -242,6 +246,11 @@ def test_plan_shapes_of_headline_configs(fl):
     assert [(q["src"], q["dst"]) for q in p] == [(1, 2), (2, 1)] and not fc
     # other depths over 256 x 256 planes keep the plane pass
     assert [q["mode"] for q in describe(fl, [128, 256, 256], 1)[0]] == [2, 1]
+    # 128^3 blocks (the differentiable 16 x 128^3 configuration): the y-split pair
+    p, fc = describe(fl, [128, 128, 128], 16)
+    assert [(q["mode"], q["lg"], q["NT"]) for q in p] == [(4, 7, 256), (5, 7, 256)] and not fc
+    assert [q["mode"] for q in describe(fl, [128, 128, 128], 16, flags=8)[0]] == [2, 1]
+    assert [q["mode"] for q in describe(fl, [64, 128, 128], 1)[0]] == [2, 1]
     # without plane fusion: three single passes (x contiguous, y and z strided)
     p, fc = describe(fl, [256, 256, 256], 1, flags=4)
     assert [q["mode"] for q in p] == [0, 1, 1] and not fc and all(q["dst"] == 1 for q in p)
