@@ -39,9 +39,9 @@ extern "C" {
 /* Single-precision arrays whose three fastest axes are 256 x 256 x 256 or
  * 128 x 128 x 128 run in two passes with the y axis split (n/4) x 4 between
  * them (x and (n/4)-point y over one residue class of rows per workgroup; then
- * the 4-point y stage and z),
- * so that every tile fits one workgroup and no input is read twice.  This
- * flag keeps the plane pass and a separate z pass instead. */
+ * the 4-point y stage and z), so that every tile fits one workgroup and no
+ * input is read twice.  This flag keeps the plane pass and a separate z pass
+ * instead. */
 #define FFTAMD_NO_CUBE_SPLIT 0x8u
 
 /* Single precision, rank 1..3, every dimension a power of two, `batch`
