@@ -1437,11 +1437,12 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
                        d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K)
                        : d.mode == MODE_COL4096 ? 16
                        : d.mode == MODE_CUBE_A ? (1LL << d.lg) / 4   // rows per workgroup
-                       : d.mode == MODE_CUBE_B ? 16   // columns per workgroup (x 4 rows x n planes)
+                       : d.mode == MODE_CUBE_B ? (d.lg == 8 ? 16 : 32)   // columns per workgroup (x 4 rows x n planes)
                        : geom_cols_for(1 << d.lg, d.mode, sp(d)),
                        d.mode == MODE_PLANE ? plane_threads(d)
                        : d.mode == MODE_COL4096 ? 512
-                       : d.mode == MODE_CUBE_A || d.mode == MODE_CUBE_B ? (d.lg == 8 ? 512 : 256)
+                       : d.mode == MODE_CUBE_A ? (d.lg == 8 ? 512 : 256)
+                       : d.mode == MODE_CUBE_B ? 512
                        : geom_threads_for(1 << d.lg, d.mode, sp(d))};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }

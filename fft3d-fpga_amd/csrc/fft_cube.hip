@@ -195,17 +195,18 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
 }
 
 // ---- 128 x 128 x 128 blocks: the same split, y = a + 4b (b < 32), ky = k + 32d.
-// Smaller tiles: pass A holds 32 rows of 128 (32 KiB) in LDS with 256 threads
-// of 16 points; pass B 16 columns x 4 rows x 128 planes (64 KiB) with 256
-// threads of 32 points.  Four workgroups per CU.
+// Pass A holds 32 rows of 128 (32 KiB) in LDS with 256 threads of 16 points
+// (four workgroups per CU); pass B 32 columns x 4 rows x 128 planes (128 KiB)
+// with 512 threads of 32 points (two per CU).
 constexpr int C1_L = 128;
 constexpr int C1_NT = 256;
 constexpr long long C1_PLANE = (long long)C1_L * C1_L;
 constexpr long long C1_BLOCK = C1_PLANE * C1_L;
 using C1Row = Geom<C1_L, MODE_CONTIG, 32>;   // pass A rows: 32 rows of 128, 8 threads each
 using C1Col = Geom<32, MODE_STRIDED, 128>;   // pass A columns: 128 columns of 32, 2 threads each
-using C1Z = Geom<C1_L, MODE_STRIDED, 32>;    // pass B: 32 lines (2 rows x 16 columns) of 128
-static_assert(C1Row::NT == C1_NT && C1Col::NT == C1_NT && C1Z::NT == C1_NT, "256 threads");
+constexpr int C1B_NT = 512;
+using C1Z = Geom<C1_L, MODE_STRIDED, 64>;    // pass B: 64 lines (2 rows x 32 columns) of 128
+static_assert(C1Row::NT == C1_NT && C1Col::NT == C1_NT && C1Z::NT == C1B_NT, "256 / 512 threads");
 static_assert(sched_np(C1_L) == 2 && sched_np(32) == 2, "two stages per line");
 constexpr int C1A_LDS = C1Row::LDS_ELEMS > C1Col::LDS_ELEMS ? C1Row::LDS_ELEMS : C1Col::LDS_ELEMS;
 
@@ -265,15 +266,18 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
   for (int r = 0; r < 16; ++r) out[(long long)(32 * res + tq + 2 * r) * C1_L + cc] = v[r];
 }
 
+// Pass B of 128^3 blocks: 32 adjacent columns (256-byte row segments, as the
+// 128-point z line pass; 16 columns measured 102.6 against 98.9 us) x 4 rows
+// x 128 planes (128 KiB) with 512 threads of 32 points, two workgroups per CU.
 template <bool INV, bool WS>
-__global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_b(const CubeArgs a) {
+__global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   __shared__ cpx<float> buf[C1Z::LDS_ELEMS];
   const unsigned b = blockIdx.x;
-  const int xg = b & 7;                // group of 16 columns
-  const int k = (b >> 3) & 31;         // rows k + 32 a
-  const unsigned long long blk = b >> 8;
+  const int xg = b & 3;                // group of 32 columns
+  const int k = (b >> 2) & 31;         // rows k + 32 a
+  const unsigned long long blk = b >> 7;
   if (blk >= a.nplanes / C1_L) return;  // uniform over the workgroup
-  const long long base = (long long)blk * C1_BLOCK + (long long)k * C1_L + 16 * xg;
+  const long long base = (long long)blk * C1_BLOCK + (long long)k * C1_L + 32 * xg;
   const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
   cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
   const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
@@ -281,7 +285,7 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   const int tid = threadIdx.x;
 
   // ---- y over a: thread (x, zs) holds planes z = zs + 16 j; t[j][d] = X[k + 32 d]
-  const int x = tid & 15, zs = tid >> 4;
+  const int x = tid & 31, zs = tid >> 5;
   cpx<float> t[8][4];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -298,25 +302,24 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_b(const CubeArgs a) {
     t[j][3] = cmul(t[j][3], w3);
     dft4(t[j]);
   }
-
-  // ---- z: the 32 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 16 (d - 2 e) + x
-  const int cz = tid & 31;
-  const int tz = tid >> 5;
+  // ---- z: the 64 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 32 (d - 2 e) + x
+  const int cz = tid & 63;
+  const int tz = tid >> 6;
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     if (e) __syncthreads();  // the first half's last stage has read the buffer
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       buf[C1Z::idx(x, zs + 16 * j)] = t[j][2 * e];
-      buf[C1Z::idx(16 + x, zs + 16 * j)] = t[j][2 * e + 1];
+      buf[C1Z::idx(32 + x, zs + 16 * j)] = t[j][2 * e + 1];
     }
     __syncthreads();
     cpx<float> v[16];
-    stage_load_lds<float, C1_L, MODE_STRIDED, 0, 32>(v, buf, cz, tz);
+    stage_load_lds<float, C1_L, MODE_STRIDED, 0, 64>(v, buf, cz, tz);
     __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
-    run_stages<float, C1_L, MODE_STRIDED, 0, false, 32>(v, buf, cz, tz, tw);
-    // point kz = tz + 8 r of row k + 32 d, column 16 xg + (cz & 15)
-    const long long ob = (long long)(32 * (2 * e + (cz >> 4))) * C1_L + (cz & 15);
+    run_stages<float, C1_L, MODE_STRIDED, 0, false, 64>(v, buf, cz, tz, tw);
+    // point kz = tz + 8 r of row k + 32 d, column 32 xg + (cz & 31)
+    const long long ob = (long long)(32 * (2 * e + (cz >> 5))) * C1_L + (cz & 31);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const long long o = ob + (long long)(tz + 8 * r) * C1_PLANE;
@@ -366,8 +369,8 @@ int launch_fft_cube_b(const CubeArgs& a, void* stream) {
   unsigned long long blocks;
   if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || a.nplanes % a.n || a.w_load) return -1;
   if (a.n == C1_L) {
-    if (!cube_grid(a.nplanes / C1_L, 256, &blocks)) return -1;
-    const dim3 g((unsigned)blocks), t(C1_NT);
+    if (!cube_grid(a.nplanes / C1_L, 128, &blocks)) return -1;
+    const dim3 g((unsigned)blocks), t(C1B_NT);
     if (a.inverse) {
       if (a.w_store) hipLaunchKernelGGL((k_fft_cube128_b<true, true>), g, t, 0, s, a);
       else hipLaunchKernelGGL((k_fft_cube128_b<true, false>), g, t, 0, s, a);

@@ -248,7 +248,7 @@ def test_plan_shapes_of_headline_configs(fl):
     assert [q["mode"] for q in describe(fl, [128, 256, 256], 1)[0]] == [2, 1]
     # 128^3 blocks (the differentiable 16 x 128^3 configuration): the y-split pair
     p, fc = describe(fl, [128, 128, 128], 16)
-    assert [(q["mode"], q["lg"], q["NT"]) for q in p] == [(4, 7, 256), (5, 7, 256)] and not fc
+    assert [(q["mode"], q["lg"], q["NT"]) for q in p] == [(4, 7, 256), (5, 7, 512)] and not fc
     assert [q["mode"] for q in describe(fl, [128, 128, 128], 16, flags=8)[0]] == [2, 1]
     assert [q["mode"] for q in describe(fl, [64, 128, 128], 1)[0]] == [2, 1]
     # without plane fusion: three single passes (x contiguous, y and z strided)
