@@ -1,34 +1,32 @@
-// fft_cube.hip -- 256 x 256 x 256 blocks in two passes over HBM with the y
-// axis split 64 x 4 between them, so that no tile is larger than one
-// workgroup's registers and LDS and no input is read twice.
+// fft_cube.hip -- n x n x n blocks (n = 256 or 128) in two passes over HBM
+// with the y axis split (n/4) x 4 between them, so that no tile is larger than
+// one workgroup's registers and LDS and no input is read twice.
 //
-// y = a + 4 b (a < 4, b < 64), ky = k + 64 d (k < 64, d < 4):
-//   X[k + 64 d] = sum_a W4^{a d} T_a[k],
-//   T_a[k]      = W256^{a k} sum_b x[a + 4 b] W64^{b k}.
+// y = a + 4 b (a < 4, b < n/4), ky = k + (n/4) d (k < n/4, d < 4):
+//   X[k + (n/4) d] = sum_a W4^{a d} T_a[k],
+//   T_a[k]         = Wn^{a k} sum_b x[a + 4 b] W(n/4)^{b k}.
 //
-// Pass A (k_fft_cube_a, one workgroup per plane and residue a): reads the 64
-// rows y = a + 4 b of a plane (128 KiB, whole 2 KiB rows), transforms each
-// along x (256 points), then each column along b (64 points), and writes the
-// column DFT (T_a[k] without its twiddle) to row 64 a + k.  Out of place: the
-// rows it writes are read by the other three residues.
+// Pass A (one workgroup per plane and residue a): reads the n/4 rows
+// y = a + 4 b of a plane (whole rows), transforms each along x, then each
+// column along b, and writes the column DFT (T_a[k] without its twiddle) to
+// row (n/4) a + k.  Out of place: the rows it writes are read by the other
+// three residues.
 //
-// Pass B (k_fft_cube_b, one workgroup per (block, k, group of 16 columns)):
-// reads rows k + 64 a, a = 0..3, of 16 adjacent columns in all 256 planes
-// (128 KiB, 128-byte row segments), multiplies row k + 64 a by W256^{a k}, forms
-// X[k + 64 d] with a 4-point DFT over a, transforms the resulting 64 lines along z (256 points) and writes them
-// back to the same rows and columns (in place allowed).  A thread loads all
-// four rows of one column in 8 planes, so the 4-point DFT is in registers, and
-// sixteen adjacent lanes read or write one 128-byte row segment.  (Loading the
-// z transform's first-stage inputs of two rows per lane instead, with the
-// 4-point DFT finished across lane pairs by a DPP swap, saves one LDS exchange
-// but interleaves two rows over adjacent lanes: measured slower, 55 against
-// 48 us forward.)
+// Pass B (one workgroup per (block, k, group of adjacent columns)): reads rows
+// k + (n/4) a, a = 0..3, of its columns in all n planes, multiplies row
+// k + (n/4) a by Wn^{a k}, forms X[k + (n/4) d] with a 4-point DFT over a,
+// transforms the resulting lines along z and writes them back to the same
+// rows and columns (in place allowed).  A thread loads all four rows of one
+// column in 8 planes, so the 4-point DFT is in registers, and adjacent lanes
+// read or write whole 128- or 256-byte row segments.
 //
-// Against the 256 x 256 plane pass, whose four workgroups per plane each read
-// the whole 512 KiB plane, every tile here fits one workgroup: 512 threads
-// hold 32 points each in registers and LDS stages half a tile at a time
-// (68 KiB), so two workgroups share a CU.  Inverse: swap(DFT(swap(x))), the
-// swaps applied at pass A's loads and pass B's stores.
+// n = 256: pass A's tile is 64 rows (128 KiB), pass B's 16 columns x 4 rows x
+// 256 planes (128 KiB); 512 threads hold 32 points each in registers and LDS
+// stages half a tile at a time (68 KiB), so two workgroups share a CU.  Against
+// the 256 x 256 plane pass, whose four workgroups per plane each read the whole
+// 512 KiB plane, nothing is read twice.  n = 128: see the second pair below.
+// Inverse: swap(DFT(swap(x))), the swaps applied at pass A's loads and pass
+// B's stores.
 #include <hip/hip_runtime.h>
 
 #include "fft_pass_dev.h"
