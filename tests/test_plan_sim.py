@@ -543,3 +543,108 @@ def test_plane256_block_mapping(nplanes):
     for p in range(min(nplanes, 16)):
         bs = b[live & (plane == p)]
         assert len(set(bs & 7)) == 1 and np.array_equal(np.diff(np.sort(bs)), [8, 8, 8])
+
+
+# ---- the y-split pair of n^3 blocks (fft_cube.hip): thread, LDS and store maps ----------
+
+def _cube_geom(n):
+    """Per-side constants of k_fft_cube_a/b (n = 256) and k_fft_cube128_a/b (n = 128)."""
+    if n == 256:
+        return dict(nt_a=512, tpl_row=16, halves=2, rows_per_half=32, ldsa_pad=129, col_tpl=4,
+                    nt_b=512, cols_b=16, zs_b=32, ldsb_pad=33, tpl_z=16)
+    return dict(nt_a=256, tpl_row=8, halves=1, rows_per_half=32, ldsa_pad=129, col_tpl=2,
+                nt_b=512, cols_b=32, zs_b=16, ldsb_pad=65, tpl_z=8)
+
+
+def emulate_cube_a(blocks, n):
+    """Pass A over planes [..., n, n]: thread (c, tau) holds point kx = tau + TPL r of
+    row b = rows_per_half h + c (y = res + 4b); the column halves go to LDS at
+    [p][col] (p = b, one pad column); column thread (cc, tq) stores point
+    k = tq + col_tpl r of its column to row (n/4) res + k."""
+    g = _cube_geom(n)
+    q = n // 4
+    planes = blocks.reshape(-1, n, n)
+    out = np.full(planes.shape, np.nan + 0j)
+    tid = np.arange(g["nt_a"])
+    c, tau = tid // g["tpl_row"], tid % g["tpl_row"]
+    r = np.arange(n // g["tpl_row"])
+    ncol_half = n // (2 if g["halves"] == 2 else 1)
+    cc, tq = tid % ncol_half, tid // ncol_half
+    for res in range(4):
+        rowfft = np.fft.fft(planes[:, res::4, :], axis=2)  # [plane, b, kx]
+        regs = []
+        for h in range(g["halves"]):
+            b = g["rows_per_half"] * h + c
+            regs.append(rowfft[:, b[:, None], (tau[:, None] + g["tpl_row"] * r[None, :])])  # [plane, tid, r]
+        for hc in range(g["halves"]):
+            lds = np.full((planes.shape[0], q * g["ldsa_pad"]), np.nan + 0j)
+            per = len(r) // g["halves"]
+            for h in range(g["halves"]):
+                for rr in range(per):
+                    col = tau + g["tpl_row"] * rr
+                    idx = (c + g["rows_per_half"] * h) * g["ldsa_pad"] + col
+                    assert np.isnan(lds[0, idx]).all(), "LDS slot written twice"
+                    lds[:, idx] = regs[h][:, :, per * hc + rr]
+            p = np.arange(q)
+            lines = lds[:, p[None, :] * g["ldsa_pad"] + np.arange(ncol_half)[:, None]]  # [plane, col, p]
+            assert not np.isnan(lines).any(), "LDS column slot left empty"
+            colfft = np.fft.fft(lines, axis=2)  # [plane, col, k]
+            kk = tq[:, None] + g["col_tpl"] * np.arange(q // g["col_tpl"])[None, :]
+            orow = q * res + kk
+            ocol = ncol_half * hc + np.broadcast_to(cc[:, None], kk.shape)
+            assert np.isnan(out[0, orow, ocol]).all(), "output written twice"
+            out[:, orow, ocol] = colfft[:, cc[:, None], kk]
+    assert not np.isnan(out).any()
+    return out.reshape(blocks.shape)
+
+
+def emulate_cube_b(arr, n):
+    """Pass B over blocks [..., n, n, n] in place: thread (x, zs) loads rows
+    k + (n/4) a of column cols xg + x in planes zs + zs_b j, applies Wn^{a k} and
+    the 4-point DFT; the lines of rows d = 2e, 2e + 1 go to LDS at [z][line]
+    (one pad column); line thread (cz, tz) stores kz = tz + TPL r."""
+    g = _cube_geom(n)
+    q, cb = n // 4, g["cols_b"]
+    blk = arr.reshape(-1, n, n, n).copy()
+    out = np.full(blk.shape, np.nan + 0j)
+    tid = np.arange(g["nt_b"])
+    x, zs = tid % cb, tid // cb
+    j = np.arange(8)
+    z = zs[:, None] + g["zs_b"] * j[None, :]  # [tid, j]
+    assert np.array_equal(np.sort(z.reshape(-1)), np.repeat(np.arange(n), cb))
+    cz, tz = tid % (2 * cb), tid // (2 * cb)
+    for k in range(q):
+        w = np.exp(-2j * np.pi * np.arange(4) * k / n)
+        for xg in range(n // cb):
+            cols = cb * xg + x
+            t = np.stack([blk[:, z, (k + q * a), cols[:, None]] for a in range(4)], axis=-1)  # [blk, tid, j, a]
+            t = np.fft.fft(t * w, axis=-1)  # d
+            for e in range(2):
+                lds = np.full((blk.shape[0], n * g["ldsb_pad"]), np.nan + 0j)
+                for dd in range(2):
+                    idx = z * g["ldsb_pad"] + (x + cb * dd)[:, None]
+                    assert np.isnan(lds[0, idx]).all(), "LDS slot written twice"
+                    lds[:, idx] = t[..., 2 * e + dd]
+                zz = np.arange(n)
+                lines = lds[:, zz[None, :] * g["ldsb_pad"] + np.arange(2 * cb)[:, None]]  # [blk, line, z]
+                assert not np.isnan(lines).any()
+                zf = np.fft.fft(lines, axis=2)
+                kz = tz[:, None] + g["tpl_z"] * np.arange(n // g["tpl_z"])[None, :]
+                orow = k + q * (2 * e + cz // cb)
+                ocol = cb * xg + cz % cb
+                sel = out[0, kz, orow[:, None], ocol[:, None]]
+                assert np.isnan(sel).all(), "output written twice"
+                out[:, kz, orow[:, None], ocol[:, None]] = zf[:, cz[:, None], kz]
+    assert not np.isnan(out).any()
+    return out.reshape(arr.shape)
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_cube_split_index_maps(n):
+    """The two kernels' maps, replayed: every LDS slot and output element is
+    written exactly once, and pass B after pass A is the 3D DFT."""
+    rng = np.random.default_rng(11 + n)
+    x = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    got = emulate_cube_b(emulate_cube_a(x, n), n)
+    ref = np.fft.fftn(x)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
