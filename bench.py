@@ -53,6 +53,7 @@ WORKLOADS = {
     "2d_4096_fwd": ([4096, 4096], 1, 1, False),
     "1d_4m_fwd": ([1 << 22], 1, 1, False),
     "3d_512_fwd": ([512, 512, 512], 1, 3, False),
+    "3d_128_b16_fwd_inv": ([128, 128, 128], 16, 4, True),
 }
 
 
@@ -460,8 +461,8 @@ def main():
 
     bitrev = rank == 1
     fusion = {"on": not args.no_plane_fusion, "cube": not args.no_cube_split}
-    fusable = rank >= 2 and dims[-1] == 256 and dims[-2] == 256
-    cubic = rank == 3 and dims == [256, 256, 256]
+    fusable = rank >= 2  # plane passes / one-pass 4096-point columns where the shape has them
+    cubic = rank == 3 and dims[0] == dims[1] == dims[2] and dims[0] in (128, 256)  # y-split pair
 
     def step():
         fl.c2c_dev(x, rank, inverse=False, out=X, bitrev_1d=bitrev, plane_fusion=fusion["on"],
