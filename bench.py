@@ -15,7 +15,7 @@ cpu_baseline = the FP64 oracle (a scalar C port, 1 thread) on one forward
          transform of the same workload, timed on this host.
 accuracy = relative L2 error of this run's forward output (first transform)
          against that oracle output, and of the round trip (y / N - x).
-Further objects on the same line (default run): plane_pass_path (the same
+Further objects on the same line (the default workload's run only): plane_pass_path (the same
 steps with the 256 x 256 plane pass and a z pass instead of the y-split
 pair), unfused_passes (the same
 steps with 256 x 256 planes as separate x and y passes), torch_fft_rocfft
@@ -594,17 +594,19 @@ def main():
         result["torch_fft_rocfft"] = {"ms_per_step": round(ms_t, 5),
                                       "gflops": round(flops / (ms_t * 1e-3) / 1e9, 2),
                                       "speedup_vs_torch": round(ms_t / ms, 3)}
-    if not args.no_batched:
+    # the other configurations ride along with the default workload only
+    default = args.workload == "3d_256_fwd_inv"
+    if default and not args.no_batched:
         result["batched"] = batched_vs_plain(max(5, min(args.steps, 20)), 2)
     del x, X, y
     torch.cuda.empty_cache()
-    if not args.no_split:
+    if default and not args.no_split:
         result["split_512"] = split_512(max(3, min(args.steps, 10)))
-    if not args.no_autograd:
+    if default and not args.no_autograd:
         result["autograd_16x128"] = autograd_16x128(max(3, min(args.steps, 10)))
-    if args.workload == "3d_256_fwd_inv" and not args.no_other:
+    if default and not args.no_other:
         result["other_workloads"] = other_workloads(max(5, min(args.steps, 50)), args.warmup)
-    if not args.no_host_api:
+    if default and not args.no_host_api:
         result["host_api_pcie"] = host_api_pcie()
     if not args.no_cpu_baseline:
         result["cpu_baseline"], err = cpu_baseline(dims, seed, fwd_first, bitrev)
