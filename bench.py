@@ -403,11 +403,10 @@ def batched_vs_plain(steps, warmup):
     flops = sum(flops_per_transform(sh) for sh in shapes)
     alg = 16.0 * total
     # the per-transform path with the same kernels as the tuned schedule
-    # (mode 2: plane kernels, also for the 128^3 members, which a single call
-    # otherwise runs through the y-split pair; modes 0, 1: line passes) must
-    # give the same bits
+    # (mode 2: plane kernels, and the y-split pair for the 128^3 members;
+    # modes 0, 1: line passes) must give the same bits
     for a, b in views:
-        fl.c2c_dev(a, 3, out=b, plane_fusion=(tune["mode"] == 2), cube_split=False)
+        fl.c2c_dev(a, 3, out=b, plane_fusion=(tune["mode"] == 2))
     torch.cuda.synchronize()
     same = bool(torch.equal(d_out, d_ref))
     return {

@@ -762,6 +762,7 @@ struct PlaneGroup {
   int ly = 0, lx = 0;
   unsigned ppt = 0, count = 0;  // planes per transform, transforms
   size_t tab_off = 0;           // first (in, out) pair of the group in the device tables
+  bool cube = false;            // 128^3 transforms whose input and output do not overlap: the y-split pair
 };
 
 struct MultiPlan {
@@ -960,6 +961,15 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
       g.lx = (int)std::get<2>(it->first);
       g.count = (unsigned)it->second.size();
       g.tab_off = tab.size() / 2;
+      // 128^3 transforms run through the y-split pair, whose first pass runs
+      // out of place: only when no transform's input and output overlap
+      g.cube = !mp->split && g.ppt == 128 && g.ly == 128 && g.lx == 128;
+      for (unsigned i : it->second) {
+        const unsigned long long bytes = 8ull * 128 * 128 * 128;
+        const unsigned long long ib = (unsigned long long)(uintptr_t)din + 8ull * in_off[i];
+        const unsigned long long ob = (unsigned long long)(uintptr_t)dout + 8ull * out_off[i];
+        if (ib < ob + bytes && ob < ib + bytes) g.cube = false;
+      }
       for (unsigned i : it->second) {
         tab.push_back(in_off[i]);
         tab.push_back(out_off[i]);
@@ -998,6 +1008,28 @@ static int multi_prepare_fork(MultiPlan* mp) {
 // x and y of one shape group as one plane-kernel launch, then its z axis as
 // one strided line pass over all its transforms, in place
 static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stream) {
+  if (g.cube) {
+    CubeArgs ca{};
+    ca.n = 128;
+    ca.nplanes = (unsigned long long)g.count * 128;
+    if (get_stockham(PREC_F32, 128, &ca.tw)) return -1;
+    if (get_direct(256, &ca.w256)) return -1;
+    ca.inverse = mp->inverse ? 1 : 0;
+    ca.in = mp->din;
+    ca.out = mp->dout;
+    ca.tab = mp->d_tab + 2 * g.tab_off;
+    if (launch_fft_cube_a(ca, stream)) {
+      set_error("y-split pass A launch failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+    ca.in = mp->dout;  // pass B in place on the output
+    ca.tab = mp->d_tab_z + 2 * g.tab_off;
+    if (launch_fft_cube_b(ca, stream)) {
+      set_error("y-split pass B launch failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+    return 0;
+  }
   PlaneArgs a{};
   a.in = mp->din;
   a.out = mp->dout;

@@ -214,9 +214,16 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
   const unsigned long long plane = blockIdx.x >> 2;
   const int res = blockIdx.x & 3;  // residue class a of the input rows
   if (plane >= a.nplanes) return;  // uniform over the workgroup
-  const long long base = (long long)plane * C1_PLANE;
-  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
-  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  const long long base = (long long)plane * C1_PLANE;  // also the weights' element index
+  long long ibase = base, obase = base;
+  if (a.tab) {  // blocks of a batch of transforms placed anywhere
+    const unsigned long long t = plane / C1_L;
+    const long long zo = (long long)(plane % C1_L) * C1_PLANE;
+    ibase = (long long)a.tab[2 * t] + zo;
+    obase = (long long)a.tab[2 * t + 1] + zo;
+  }
+  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + ibase;
+  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + obase;
   const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
   const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
   const int tid = threadIdx.x;
@@ -275,9 +282,12 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   const int k = (b >> 2) & 31;         // rows k + 32 a
   const unsigned long long blk = b >> 7;
   if (blk >= a.nplanes / C1_L) return;  // uniform over the workgroup
-  const long long base = (long long)blk * C1_BLOCK + (long long)k * C1_L + 32 * xg;
-  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + base;
-  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
+  const long long tile = (long long)k * C1_L + 32 * xg;
+  const long long base = (long long)blk * C1_BLOCK + tile;  // also the weights' element index
+  const long long ibase = a.tab ? (long long)a.tab[2 * blk] + tile : base;
+  const long long obase = a.tab ? (long long)a.tab[2 * blk + 1] + tile : base;
+  const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + ibase;
+  cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + obase;
   const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
   const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
   const int tid = threadIdx.x;
@@ -338,7 +348,8 @@ static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsign
 int launch_fft_cube_a(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
-  if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || !cube_grid(a.nplanes, 4, &blocks) || a.w_store)
+  if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || !cube_grid(a.nplanes, 4, &blocks) || a.w_store ||
+      (a.tab && (a.n != C1_L || a.w_load)))
     return -1;
   if (a.n == C1_L) {
     const dim3 g((unsigned)blocks), t(C1_NT);
@@ -365,7 +376,9 @@ int launch_fft_cube_a(const CubeArgs& a, void* stream) {
 int launch_fft_cube_b(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
-  if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || a.nplanes % a.n || a.w_load) return -1;
+  if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || a.nplanes % a.n || a.w_load ||
+      (a.tab && (a.n != C1_L || a.w_store)))
+    return -1;
   if (a.n == C1_L) {
     if (!cube_grid(a.nplanes / C1_L, 128, &blocks)) return -1;
     const dim3 g((unsigned)blocks), t(C1B_NT);

@@ -77,6 +77,10 @@ struct CubeArgs {
   const float* w_store;
   unsigned long long w_mask;
   int inverse;
+  // optional transform table (n = 128 only, no weights): block t starts at
+  // tab[2 t] in `in` and tab[2 t + 1] in `out` (complex elements); NULL:
+  // block t at t n^3 in both
+  const unsigned long long* tab;
 };
 int launch_fft_cube_a(const CubeArgs& a, void* stream);
 int launch_fft_cube_b(const CubeArgs& a, void* stream);

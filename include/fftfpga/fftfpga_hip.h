@@ -109,7 +109,8 @@ int fftamd_multi_tune(fftamd_multi_t plan, void *stream, const char *cache_path,
  * of at most 128 x 128, single precision without split): per shape group, the
  * x and y passes as one plane-kernel launch (a whole plane per workgroup, in
  * LDS) and the z passes as one line-pass launch, both over a table of the
- * group's transforms (grid and chunk do not apply).  The tuner considers
+ * group's transforms (128^3 groups whose inputs and outputs do not overlap:
+ * the y-split pair instead; grid and chunk do not apply).  The tuner considers
  * every available mode. */
 int fftamd_multi_schedule(fftamd_multi_t plan, int *mode, unsigned *grid, unsigned *chunk,
                           unsigned *items, unsigned *phases);

@@ -74,7 +74,8 @@ def test_mixed_batch_in_place(gpu_lib, orc):
         assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s))) <= TOL
 
 
-PLANES = [(32, 32, 32)] * 5 + [(64, 64, 64)] * 3 + [(16, 128, 128)] * 2 + [(128, 64, 32), (64, 32, 128)]
+PLANES = [(32, 32, 32)] * 5 + [(64, 64, 64)] * 3 + [(16, 128, 128)] * 2 + [(128, 64, 32), (64, 32, 128)] \
+    + [(128, 128, 128)] * 2  # out of place: the y-split pair through the transform table
 
 
 @pytest.mark.parametrize("inverse", [False, True])
@@ -147,8 +148,9 @@ def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode):
     ref = torch.empty_like(d_in)
     for s, o in zip(shapes, offs):
         n = int(np.prod(s))
-        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s), plane_fusion=(mode == 2),
-                cube_split=False)  # mode 2 runs plane kernels, also for the 128^3 members
+        # mode 2 (out of place) runs the same kernels as a single call: plane kernels for
+        # 32^3 and 64^3, the y-split pair for 128^3
+        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s), plane_fusion=(mode == 2))
     torch.cuda.synchronize()
     assert torch.equal(d_out, ref)
     out = d_out.cpu().numpy()
