@@ -559,7 +559,8 @@ def _cube_geom(n):
 def emulate_cube_a(blocks, n):
     """Pass A over planes [..., n, n]: thread (c, tau) holds point kx = tau + TPL r of
     row b = rows_per_half h + c (y = res + 4b); the column halves go to LDS at
-    [p][col] (p = b, one pad column); column thread (cc, tq) stores point
+    [row][col] (one pad column) -- for n = 256 as the radix-2 partial sums of the
+    4-point stage, for n = 128 as is; column thread (cc, tq) stores point
     k = tq + col_tpl r of its column to row (n/4) res + k."""
     g = _cube_geom(n)
     q = n // 4
@@ -570,8 +571,12 @@ def emulate_cube_a(blocks, n):
     r = np.arange(n // g["tpl_row"])
     ncol_half = n // (2 if g["halves"] == 2 else 1)
     cc, tq = tid % ncol_half, tid // ncol_half
+
+    def dft_a(v, ax):
+        return np.fft.fft(v, axis=ax)
+
     for res in range(4):
-        rowfft = np.fft.fft(planes[:, res::4, :], axis=2)  # [plane, b, kx]
+        rowfft = dft_a(planes[:, res::4, :], 2)  # [plane, b, kx]
         regs = []
         for h in range(g["halves"]):
             b = g["rows_per_half"] * h + c
@@ -579,16 +584,37 @@ def emulate_cube_a(blocks, n):
         for hc in range(g["halves"]):
             lds = np.full((planes.shape[0], q * g["ldsa_pad"]), np.nan + 0j)
             per = len(r) // g["halves"]
-            for h in range(g["halves"]):
+            p = np.arange(q)
+            if n == 256:
+                # b = b0 + 16 (lo + 2 hi); thread c holds b0 = c & 15, lo = c >> 4,
+                # hi = 0 (r0) and 1 (r1); E_lo[m] = r0 + (-1)^m r1 to row b0 + 16 (m + 2 lo)
+                b0, lo = c & 15, c >> 4
                 for rr in range(per):
                     col = tau + g["tpl_row"] * rr
-                    idx = (c + g["rows_per_half"] * h) * g["ldsa_pad"] + col
-                    assert np.isnan(lds[0, idx]).all(), "LDS slot written twice"
-                    lds[:, idx] = regs[h][:, :, per * hc + rr]
-            p = np.arange(q)
-            lines = lds[:, p[None, :] * g["ldsa_pad"] + np.arange(ncol_half)[:, None]]  # [plane, col, p]
-            assert not np.isnan(lines).any(), "LDS column slot left empty"
-            colfft = np.fft.fft(lines, axis=2)  # [plane, col, k]
+                    u, w = regs[0][:, :, per * hc + rr], regs[1][:, :, per * hc + rr]
+                    for m, val in ((0, u + w), (1, u - w)):
+                        idx = (b0 + 16 * (m + 2 * lo)) * g["ldsa_pad"] + col
+                        assert np.isnan(lds[0, idx]).all(), "LDS slot written twice"
+                        lds[:, idx] = val
+                rows_ = lds[:, p[None, :] * g["ldsa_pad"] + np.arange(ncol_half)[:, None]]  # [plane, col, row]
+                assert not np.isnan(rows_).any(), "LDS slot left empty"
+                colfft = np.empty_like(rows_)
+                sgn = -1
+                for k1 in range(4):  # X4 = E_0[m] + W4^{k1} E_1[m], twiddle W64^{b0 k1}, DFT16 over b0
+                    m = k1 & 1
+                    x4 = rows_[:, :, 16 * m:16 * m + 16] + np.exp(sgn * 2j * np.pi * k1 / 4) * rows_[:, :, 16 * (m + 2):16 * (m + 2) + 16]
+                    x4 = x4 * np.exp(sgn * 2j * np.pi * k1 * np.arange(16) / 64)
+                    colfft[:, :, k1::4] = dft_a(x4, 2)
+            else:
+                for h in range(g["halves"]):
+                    for rr in range(per):
+                        col = tau + g["tpl_row"] * rr
+                        idx = (c + g["rows_per_half"] * h) * g["ldsa_pad"] + col
+                        assert np.isnan(lds[0, idx]).all(), "LDS slot written twice"
+                        lds[:, idx] = regs[h][:, :, per * hc + rr]
+                lines = lds[:, p[None, :] * g["ldsa_pad"] + np.arange(ncol_half)[:, None]]  # [plane, col, p]
+                assert not np.isnan(lines).any(), "LDS column slot left empty"
+                colfft = dft_a(lines, 2)  # [plane, col, k]
             kk = tq[:, None] + g["col_tpl"] * np.arange(q // g["col_tpl"])[None, :]
             orow = q * res + kk
             ocol = ncol_half * hc + np.broadcast_to(cc[:, None], kk.shape)
