@@ -173,22 +173,35 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
     dft4(t[j]);
   }
 
-  // ---- z: the 32 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 16 (d - 2 e) + x
+  // ---- z: the 32 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 16 (d - 2 e) + x.
+  // z = zq + 16 r with r = ep + 2 j: the first z stage's radix-16 over r is an
+  // 8-point DFT over j, which a thread holding planes zs + 32 j (zs = zq + 16 ep)
+  // does in registers, P_ep[zq][m]; the stage-1 thread for k0 finishes it as
+  // Y[zq][k0] = P_0[zq][k0 & 7] + W16^{k0} P_1[zq][k0 & 7] while reading LDS, so
+  // the z transform needs one LDS exchange
+  const int zq = zs & 15, ep = zs >> 4;
   const int cz = tid & 31;
-  const int tz = tid >> 5;
+  const int tz = tid >> 5;                 // k0
+  const cpx<float> w16 = wdir[16 * tz];    // W16^{k0}
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     if (e) __syncthreads();  // the first half's last stage has read the buffer
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      buf[CbZ::idx(x, zs + 32 * j)] = t[j][2 * e];
-      buf[CbZ::idx(16 + x, zs + 32 * j)] = t[j][2 * e + 1];
+    for (int dd = 0; dd < 2; ++dd) {
+      cpx<float> p8[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) p8[j] = t[j][2 * e + dd];
+      dft8(p8);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) buf[CbZ::idx(16 * dd + x, zq + 16 * ep + 32 * m)] = p8[m];
     }
     __syncthreads();
     cpx<float> v[16];
-    stage_load_lds<float, CB_L, MODE_STRIDED, 0, 32>(v, buf, cz, tz);
-    __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
-    run_stages<float, CB_L, MODE_STRIDED, 0, false, 32>(v, buf, cz, tz, tw256);
+    const int m = tz & 7;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      v[q] = cadd(buf[CbZ::idx(cz, q + 32 * m)], cmul(buf[CbZ::idx(cz, q + 16 + 32 * m)], w16));
+    stage_compute<float, CB_L, MODE_STRIDED, 1, false, 32>(v, tz, tw256);  // W256^{k0 zq}, DFT over zq
     // point kz = tz + 16 r of row k + 64 d, column 16 xg + (cz & 15)
     const long long ob = (long long)(64 * (2 * e + (cz >> 4))) * CB_L + (cz & 15);
 #pragma unroll
@@ -321,22 +334,30 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
     t[j][3] = cmul(t[j][3], w3);
     dft4(t[j]);
   }
-  // ---- z: the 64 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 32 (d - 2 e) + x
+  // ---- z: the 64 lines of rows d = 2 e, 2 e + 1 at a time; line cz = 32 (d - 2 e) + x.
+  // A thread's planes zs + 16 j are the first z stage's radix-8 inputs
+  // (points j0 + 16 r, j0 = zs): that stage runs in registers and its outputs
+  // go to LDS at their Stockham positions 8 zs + r, so the z transform needs
+  // one LDS exchange
+  static_assert(sched_radix(C1_L, 0) == 8 && sched_ns(C1_L, 1) == 8, "radix 8, then radix 16");
   const int cz = tid & 63;
   const int tz = tid >> 6;
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     if (e) __syncthreads();  // the first half's last stage has read the buffer
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      buf[C1Z::idx(x, zs + 16 * j)] = t[j][2 * e];
-      buf[C1Z::idx(32 + x, zs + 16 * j)] = t[j][2 * e + 1];
+    for (int dd = 0; dd < 2; ++dd) {
+      cpx<float> p8[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) p8[j] = t[j][2 * e + dd];
+      dft8(p8);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) buf[C1Z::idx(32 * dd + x, 8 * zs + r)] = p8[r];
     }
     __syncthreads();
     cpx<float> v[16];
-    stage_load_lds<float, C1_L, MODE_STRIDED, 0, 64>(v, buf, cz, tz);
-    __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
-    run_stages<float, C1_L, MODE_STRIDED, 0, false, 64>(v, buf, cz, tz, tw);
+    stage_load_lds<float, C1_L, MODE_STRIDED, 1, 64>(v, buf, cz, tz);
+    stage_compute<float, C1_L, MODE_STRIDED, 1, false, 64>(v, tz, tw);
     // point kz = tz + 8 r of row k + 32 d, column 32 xg + (cz & 31)
     const long long ob = (long long)(32 * (2 * e + (cz >> 5))) * C1_L + (cz & 31);
 #pragma unroll
@@ -351,7 +372,7 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
 
 static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsigned long long* blocks) {
   const unsigned long long b = nplanes * per;
-  if (b > 0xFFFFFFFFull / CB_NT) return false;  // (512 threads bounds both kernels' grids)  // grid size in work-items is 32-bit
+  if (b > 0xFFFFFFFFull / CB_NT) return false;  // grid size in work-items is 32-bit (512 threads bounds every kernel here)
   *blocks = b;
   return true;
 }

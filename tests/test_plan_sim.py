@@ -627,8 +627,11 @@ def emulate_cube_a(blocks, n):
 def emulate_cube_b(arr, n):
     """Pass B over blocks [..., n, n, n] in place: thread (x, zs) loads rows
     k + (n/4) a of column cols xg + x in planes zs + zs_b j, applies Wn^{a k} and
-    the 4-point DFT; the lines of rows d = 2e, 2e + 1 go to LDS at [z][line]
-    (one pad column); line thread (cz, tz) stores kz = tz + TPL r."""
+    the 4-point DFT, and does an 8-point DFT over j for each line of rows
+    d = 2e, 2e + 1 into LDS ([slot][line], one pad column): for n = 256 the
+    first half of the z transform's radix-16 stage, for n = 128 its radix-8
+    stage; line thread (cz, tz) finishes the z transform and stores
+    kz = tz + TPL r."""
     g = _cube_geom(n)
     q, cb = n // 4, g["cols_b"]
     blk = arr.reshape(-1, n, n, n).copy()
@@ -648,13 +651,28 @@ def emulate_cube_b(arr, n):
             for e in range(2):
                 lds = np.full((blk.shape[0], n * g["ldsb_pad"]), np.nan + 0j)
                 for dd in range(2):
-                    idx = z * g["ldsb_pad"] + (x + cb * dd)[:, None]
+                    # the thread's 8-point DFT over j (planes zs + zs_b j) to its LDS slots
+                    p8 = np.fft.fft(t[..., 2 * e + dd], axis=2)  # [blk, tid, m]
+                    if n == 256:  # slot zq + 16 ep + 32 m, zs = zq + 16 ep
+                        slot = zs[:, None] + 32 * np.arange(8)[None, :]
+                    else:  # Stockham stage-0 position 8 zs + r
+                        slot = 8 * zs[:, None] + np.arange(8)[None, :]
+                    idx = slot * g["ldsb_pad"] + (x + cb * dd)[:, None]
                     assert np.isnan(lds[0, idx]).all(), "LDS slot written twice"
-                    lds[:, idx] = t[..., 2 * e + dd]
+                    lds[:, idx] = p8
                 zz = np.arange(n)
-                lines = lds[:, zz[None, :] * g["ldsb_pad"] + np.arange(2 * cb)[:, None]]  # [blk, line, z]
-                assert not np.isnan(lines).any()
-                zf = np.fft.fft(lines, axis=2)
+                lines = lds[:, zz[None, :] * g["ldsb_pad"] + np.arange(2 * cb)[:, None]]  # [blk, line, slot]
+                assert not np.isnan(lines).any(), "LDS slot left empty"
+                zf = np.empty_like(lines)
+                nk0 = g["tpl_z"]
+                for k0 in range(nk0):  # the stage-1 thread for k0: its 16 inputs, twiddles, DFT16
+                    if n == 256:
+                        m = k0 & 7
+                        v = lines[:, :, 32 * m:32 * m + 16] + np.exp(-2j * np.pi * k0 / 16) * lines[:, :, 32 * m + 16:32 * m + 32]
+                    else:
+                        v = lines[:, :, k0::8]
+                    v = v * np.exp(-2j * np.pi * k0 * np.arange(16) / n)
+                    zf[:, :, k0::nk0] = np.fft.fft(v, axis=2)
                 kz = tz[:, None] + g["tpl_z"] * np.arange(n // g["tpl_z"])[None, :]
                 orow = k + q * (2 * e + cz // cb)
                 ocol = cb * xg + cz % cb
