@@ -106,27 +106,39 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
   // b = b0 + 16 b1, b1 = lo + 2 hi: thread c holds b0 = c & 15, lo = c >> 4 and
   // both hi (rows c and c + 32)
   const int b0 = c & 15, lo = c >> 4;
-  const cpx<float> w4 = wdir[64 * tq];  // W4^{tq}, uniform over the wave
 #pragma unroll
   for (int hc = 0; hc < 2; ++hc) {
     __syncthreads();  // the buffer's previous contents have been read
-    // E_lo[m] = x[lo] + (-1)^m x[lo + 2] to LDS row b0 + 16 (m + 2 lo)
+    // E_lo[m] = x[lo] + (-1)^m x[lo + 2] to LDS row b0 + 16 (m + 2 lo), E_1[1]
+    // times -i (lo is uniform over the wave)
+    if (lo == 0) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const cpx<float> u = r0[8 * hc + r], w = r1[8 * hc + r];
-      buf[CaCol::idx(tau + 16 * r, b0 + 16 * (2 * lo))] = cadd(u, w);
-      buf[CaCol::idx(tau + 16 * r, b0 + 16 * (2 * lo + 1))] = csub(u, w);
+      for (int r = 0; r < 8; ++r) {
+        const cpx<float> u = r0[8 * hc + r], w = r1[8 * hc + r];
+        buf[CaCol::idx(tau + 16 * r, b0)] = cadd(u, w);
+        buf[CaCol::idx(tau + 16 * r, b0 + 16)] = csub(u, w);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const cpx<float> u = r0[8 * hc + r], w = r1[8 * hc + r];
+        buf[CaCol::idx(tau + 16 * r, b0 + 32)] = cadd(u, w);
+        buf[CaCol::idx(tau + 16 * r, b0 + 48)] = cpx<float>{u.y - w.y, w.x - u.x};
+      }
     }
     __syncthreads();
     // thread (cc, tq), k1 = tq: X4[b0] = E_0[m] + W4^{k1} E_1[m], m = k1 & 1, then
-    // W64^{k1 b0} and the 16-point DFT over b0
+    // W64^{k1 b0} and the 16-point DFT over b0.
+    // W4^{tq} E_1[m] = +-E'_1[m] with E'_1[1] = -i E_1[1] (stored so by the
+    // writer), the sign uniform over the wave
     cpx<float> v[16];
     const int m = tq & 1;
+    const float sg = (tq & 2) ? -1.f : 1.f;
 #pragma unroll
     for (int bb = 0; bb < 16; ++bb) {
       const cpx<float> e0 = buf[CaCol::idx(cc, bb + 16 * m)];
       const cpx<float> e1 = buf[CaCol::idx(cc, bb + 16 * (m + 2))];
-      v[bb] = cadd(e0, cmul(e1, w4));
+      v[bb] = {__builtin_fmaf(sg, e1.x, e0.x), __builtin_fmaf(sg, e1.y, e0.y)};  // exact: e0 +- e1
     }
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[4 * tq * r]);

@@ -587,12 +587,14 @@ def emulate_cube_a(blocks, n):
             p = np.arange(q)
             if n == 256:
                 # b = b0 + 16 (lo + 2 hi); thread c holds b0 = c & 15, lo = c >> 4,
-                # hi = 0 (r0) and 1 (r1); E_lo[m] = r0 + (-1)^m r1 to row b0 + 16 (m + 2 lo)
+                # hi = 0 (r0) and 1 (r1); E_lo[m] = r0 + (-1)^m r1 to row b0 + 16 (m + 2 lo),
+                # E_1[1] stored times -i
                 b0, lo = c & 15, c >> 4
                 for rr in range(per):
                     col = tau + g["tpl_row"] * rr
                     u, w = regs[0][:, :, per * hc + rr], regs[1][:, :, per * hc + rr]
-                    for m, val in ((0, u + w), (1, u - w)):
+                    rot = np.where(lo == 1, -1j, 1)[None, :]
+                    for m, val in ((0, u + w), (1, (u - w) * rot)):
                         idx = (b0 + 16 * (m + 2 * lo)) * g["ldsa_pad"] + col
                         assert np.isnan(lds[0, idx]).all(), "LDS slot written twice"
                         lds[:, idx] = val
@@ -600,9 +602,9 @@ def emulate_cube_a(blocks, n):
                 assert not np.isnan(rows_).any(), "LDS slot left empty"
                 colfft = np.empty_like(rows_)
                 sgn = -1
-                for k1 in range(4):  # X4 = E_0[m] + W4^{k1} E_1[m], twiddle W64^{b0 k1}, DFT16 over b0
+                for k1 in range(4):  # X4 = E_0[m] +- E'_1[m] (minus for k1 >= 2), twiddle W64^{b0 k1}, DFT16 over b0
                     m = k1 & 1
-                    x4 = rows_[:, :, 16 * m:16 * m + 16] + np.exp(sgn * 2j * np.pi * k1 / 4) * rows_[:, :, 16 * (m + 2):16 * (m + 2) + 16]
+                    x4 = rows_[:, :, 16 * m:16 * m + 16] + (-1 if k1 & 2 else 1) * rows_[:, :, 16 * (m + 2):16 * (m + 2) + 16]
                     x4 = x4 * np.exp(sgn * 2j * np.pi * k1 * np.arange(16) / 64)
                     colfft[:, :, k1::4] = dft_a(x4, 2)
             else:
