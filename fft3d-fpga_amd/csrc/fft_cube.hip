@@ -247,6 +247,7 @@ constexpr int C1A_LDS = C1Row::LDS_ELEMS > C1Col::LDS_ELEMS ? C1Row::LDS_ELEMS :
 template <bool INV, bool WL>
 __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
   __shared__ cpx<float> buf[C1A_LDS];
+  __shared__ cpx<float> tws[8 * 15];  // row stage-1 twiddles W128^{k r} at [15 k + r - 1]
   const unsigned long long plane = blockIdx.x >> 2;
   const int res = blockIdx.x & 3;  // residue class a of the input rows
   if (plane >= a.nplanes) return;  // uniform over the workgroup
@@ -260,9 +261,9 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
   }
   const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + ibase;
   cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + obase;
-  const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
   const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
   const int tid = threadIdx.x;
+  if (tid < 8 * 15) tws[tid] = wdir[2 * (tid / 15) * (tid % 15 + 1)];  // read after the first barrier
 
   // ---- x: thread (c, tau) transforms row b = c (y = res + 4c); its stage-0
   // inputs come straight from HBM; it ends with point kx = tau + 8 r
@@ -281,8 +282,16 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
         v[bb * RAD + r] = INV ? cswap(s) : s;
       }
   }
-  static_assert(sched_ns(C1_L, 1) == C1Row::TPL, "last row stage: point tau + 8 r");
-  run_stages<float, C1_L, MODE_CONTIG, 0, false, 32>(v, buf, c, tau, tw);
+  static_assert(sched_np(C1_L) == 2 && sched_ns(C1_L, 1) == C1Row::TPL, "radix 8, then radix 16 ending with point tau + 8 r");
+  stage_compute<float, C1_L, MODE_CONTIG, 0, false, 32>(v, tau, nullptr);  // radix 8, no twiddles
+  stage_store_lds<float, C1_L, MODE_CONTIG, 0, 32>(v, buf, c, tau);
+  __syncthreads();
+  stage_load_lds<float, C1_L, MODE_CONTIG, 1, 32>(v, buf, c, tau);
+  // radix-16 stage over sub-transforms of span 8: twiddles W128^{tau r} from
+  // the LDS table (lanes of one tau read one address)
+#pragma unroll
+  for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * tau + r - 1]);
+  dft16(v);
 
   // ---- y over b: 128 columns of 32 points; thread (cc, tq) ends with point
   // k = tq + 2 r of its column
@@ -324,7 +333,6 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   const long long obase = a.tab ? (long long)a.tab[2 * blk + 1] + tile : base;
   const cpx<float>* in = reinterpret_cast<const cpx<float>*>(a.in) + ibase;
   cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + obase;
-  const cpx<float>* __restrict__ tw = reinterpret_cast<const cpx<float>*>(a.tw);
   const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
   const int tid = threadIdx.x;
 
@@ -353,7 +361,7 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   // one LDS exchange
   static_assert(sched_radix(C1_L, 0) == 8 && sched_ns(C1_L, 1) == 8, "radix 8, then radix 16");
   const int cz = tid & 63;
-  const int tz = tid >> 6;
+  const int tz = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform over each wave
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     if (e) __syncthreads();  // the first half's last stage has read the buffer
@@ -369,7 +377,11 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
     __syncthreads();
     cpx<float> v[16];
     stage_load_lds<float, C1_L, MODE_STRIDED, 1, 64>(v, buf, cz, tz);
-    stage_compute<float, C1_L, MODE_STRIDED, 1, false, 64>(v, tz, tw);
+    // radix-16 stage over sub-transforms of span 8: twiddles W128^{tz r} =
+    // W256^{2 tz r}, uniform over the wave (scalar loads)
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[2 * tz * r]);
+    dft16(v);
     // point kz = tz + 8 r of row k + 32 d, column 32 xg + (cz & 31)
     const long long ob = (long long)(32 * (2 * e + (cz >> 5))) * C1_L + (cz & 31);
 #pragma unroll
