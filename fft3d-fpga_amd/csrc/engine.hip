@@ -822,6 +822,8 @@ static int multi_default_grid(unsigned* grid, unsigned* per_cu_max, bool split) 
   return 0;
 }
 
+static int multi_prepare_tables(const MultiPlan* mp);
+
 int multi_create(void** handle, unsigned count, const unsigned* dims3, const unsigned long long* in_off,
                  const unsigned long long* out_off, const void* din, void* dout, bool inverse,
                  unsigned flags) {
@@ -983,6 +985,8 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
     HIP_TRY(hipMalloc(&mp->d_tab_z, sizeof(unsigned long long) * tab_z.size()));
     HIP_TRY(hipMemcpy(mp->d_tab_z, tab_z.data(), sizeof(unsigned long long) * tab_z.size(),
                       hipMemcpyHostToDevice));
+    // made now, so that a mode-2 launch allocates nothing (it may run under capture)
+    if (multi_prepare_tables(mp.get())) return -1;
   }
   unsigned per = 0;
   if (multi_default_grid(&mp->grid, &per, mp->split)) return -1;
@@ -990,8 +994,24 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
   return 0;
 }
 
-// the streams and events of the mode-2 fork (created outside any capture)
+// every twiddle table the mode-2 launches read (cached; made outside any capture)
+static int multi_prepare_tables(const MultiPlan* mp) {
+  const void* t = nullptr;
+  for (const PlaneGroup& g : mp->groups) {
+    if (g.cube) {
+      if (get_stockham(PREC_F32, 128, &t) || get_direct(256, &t)) return -1;
+      continue;
+    }
+    if (sched_np(g.lx) > 1 && get_stockham(PREC_F32, g.lx, &t)) return -1;
+    if (sched_np(g.ly) > 1 && get_stockham(PREC_F32, g.ly, &t)) return -1;
+    if (sched_np((int)g.ppt) > 1 && get_stockham(PREC_F32, (int)g.ppt, &t)) return -1;
+  }
+  return 0;
+}
+
+// the streams and events of the mode-2 fork and its tables (created outside any capture)
 static int multi_prepare_fork(MultiPlan* mp) {
+  if (multi_prepare_tables(mp)) return -1;
   if (mp->groups.size() < 2 || !mp->aux.empty()) return 0;
   HIP_TRY(hipEventCreateWithFlags(&mp->fork_ev, hipEventDisableTiming));
   for (size_t i = 0; i < mp->groups.size(); ++i) {

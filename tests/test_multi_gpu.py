@@ -232,6 +232,50 @@ def test_graph_plane_schedule_pair_replays(gpu_lib):
     inv.close()
 
 
+def test_graph_capture_before_any_launch(gpu_lib):
+    """A mode-2 pair captured before either plan ever ran: the capture must not
+    allocate (every table a mode-2 launch reads is made when the plan is
+    created).  The library is reset first so that no table is cached from
+    earlier work.  Replays equal the eager pair run afterwards."""
+    import torch
+    from fftfpga_amd.multi import Graph, MultiPlan
+    gpu_lib.fpga_final()  # drops every cached twiddle table
+    assert gpu_lib.fpga_initialize("AMD HIP", gpu_lib.LIB_PATH, False) == 0
+    shapes = [(32, 32, 32)] * 4 + [(64, 64, 64)] * 2 + [(128, 128, 128)] * 2
+    host, d_in, offs = _pack(shapes, 12)
+    mid = torch.empty_like(d_in)
+    back = torch.empty_like(d_in)
+    fwd = MultiPlan(shapes, d_in, mid)
+    inv = MultiPlan(shapes, mid, back, inverse=True)
+    for p in (fwd, inv):
+        sch = p.schedule()
+        p.set_schedule(2, sch["grid"], sch["chunk"])
+    g = Graph([fwd, inv])  # nothing has run yet
+    replays = []
+    for _ in range(2):
+        back.zero_()
+        torch.cuda.synchronize()
+        g()
+        torch.cuda.synchronize()
+        replays.append((mid.clone(), back.clone()))
+    mid.zero_()
+    back.zero_()
+    fwd()
+    inv()
+    inv.check()
+    for r, (m, b) in enumerate(replays):
+        assert torch.equal(m, mid), f"replay {r}: forward differs from the eager result"
+        assert torch.equal(b, back), f"replay {r}: inverse differs from the eager result"
+    out = back.cpu().numpy()
+    for s, o in zip(shapes, offs):
+        n = int(np.prod(s))
+        err = np.linalg.norm(out[o:o + n] / n - host[o:o + n]) / np.linalg.norm(host[o:o + n])
+        assert err <= TOL
+    g.close()
+    fwd.close()
+    inv.close()
+
+
 def test_plane_schedule_single_group(gpu_lib, orc):
     """Mode 2 with one shape group (no stream fork) against the oracle."""
     import torch
