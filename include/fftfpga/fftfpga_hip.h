@@ -12,6 +12,11 @@
  * Layout conventions are those of fftfpga.h: interleaved complex, row-major,
  * last dimension contiguous, transforms of a batch contiguous.
  * d_in == d_out (in place) is allowed.
+ *
+ * The first call for a shape (dims, batch, flags, precision) builds its plan
+ * and twiddle tables; later calls for it allocate nothing and may be captured
+ * into a HIP graph (stream capture) and replayed.  Multi-transform plans make
+ * their tables when created.
  */
 #ifndef FFTFPGA_HIP_H
 #define FFTFPGA_HIP_H

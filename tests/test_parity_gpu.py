@@ -493,3 +493,40 @@ def test_cube_split_unaligned_input(gpu_lib, orc):
     y = c2c_dev(xt, 3)
     torch.cuda.synchronize()
     assert orc.rel_l2(y.cpu().numpy().reshape(-1), orc.fftnd(x, [256, 256, 256])) <= TOL
+
+
+@pytest.mark.parametrize("dims,batch", [((64, 64, 64), 1), ((128, 128, 128), 2), ((256, 256, 256), 1),
+                                        ((4096, 4096), 1), ((1 << 22,), 1)])
+def test_c2c_dev_pair_replays_from_cuda_graph(gpu_lib, orc, dims, batch):
+    """A forward/inverse pair of fftamd_c2c_dev captured into a torch CUDA graph
+    after one eager call per shape (the call that builds the plan and its
+    tables): the captured launches allocate nothing, and every replay equals
+    the eager pair, with eager work between the replays."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    rank = len(dims)
+    n = int(np.prod(dims)) * batch
+    x = torch.from_numpy(orc.uniform(n, 47)).cuda().view(batch, *dims)
+    X = torch.empty_like(x)
+    y = torch.empty_like(x)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        c2c_dev(x, rank, out=X)
+        c2c_dev(X, rank, inverse=True, out=y)
+    torch.cuda.synchronize()
+    eager_X, eager_y = X.clone(), y.clone()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        c2c_dev(x, rank, out=X)
+        c2c_dev(X, rank, inverse=True, out=y)
+    for r in range(3):
+        X.zero_()
+        y.mul_(0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(X, eager_X), f"replay {r}: forward differs from the eager result"
+        assert torch.equal(y, eager_y), f"replay {r}: inverse differs from the eager result"
+    ref = x.cpu().numpy().reshape(-1)
+    assert orc.rel_l2(y.cpu().numpy().reshape(-1) / (n // batch), ref) <= TOL
