@@ -661,6 +661,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       ca.ngroups = (unsigned)(p.ncol / 16);
       ca.nitems = p.nouter * ca.ngroups;
       if (get_stockham(PREC_F32, 256, &ca.tw)) return -1;
+      if (get_direct(256, &ca.w256)) return -1;
       if (get_fourstep(PREC_F32, 4096, &ca.ptw_lo, &ca.ptw_hi)) return -1;
       ca.ptw_hb = fourstep_hb(4096);
       ca.w_load = pi == 0 ? w_load : nullptr;

@@ -93,7 +93,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(const void* base, uns
 // rows k = k1 + 16 k2.  Thread (line c = 16 l + c16, tau).
 template <int Q, bool INV, bool WS>
 __device__ __forceinline__ void col_stage2(const ColArgs& a, __amdgpu_buffer_rsrc_t rout,
-                                           __amdgpu_buffer_rsrc_t rws, int h, cpx<float>* lds) {
+                                           __amdgpu_buffer_rsrc_t rws, int h, cpx<float>* lds,
+                                           const cpx<float>* tws) {
   const int tid = threadIdx.x;
   const int c = tid % ColGeom::C, tau = tid / ColGeom::C;
   const int k1 = Q + 4 * (2 * h + (c >> 4));
@@ -107,28 +108,19 @@ __device__ __forceinline__ void col_stage2(const ColArgs& a, __amdgpu_buffer_rsr
     const cpx<float> base = cmul(hi[t >> a.ptw_hb], lo[t & ((1u << a.ptw_hb) - 1u)]);
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = cmul(v[r], base);
-    const cpx<float>* tw = reinterpret_cast<const cpx<float>*>(a.tw) + sched_twoff(CL_N1, 1) + TW_PER_K * k1;
-    const cpx<float> w1 = tw[0], w2 = tw[1], w4 = tw[2], w8 = tw[3];
-    const cpx<float> w3 = cmul(w1, w2), w12 = cmul(w4, w8);
-    v[1] = cmul(v[1], w1);
-    v[2] = cmul(v[2], w2);
-    v[3] = cmul(v[3], w3);
-    v[4] = cmul(v[4], w4);
-    v[5] = cmul(v[5], cmul(w4, w1));
-    v[6] = cmul(v[6], cmul(w4, w2));
-    v[7] = cmul(v[7], cmul(w4, w3));
-    v[8] = cmul(v[8], w8);
-    v[9] = cmul(v[9], cmul(w8, w1));
-    v[10] = cmul(v[10], cmul(w8, w2));
-    v[11] = cmul(v[11], cmul(w8, w3));
-    v[12] = cmul(v[12], w12);
-    v[13] = cmul(v[13], cmul(w12, w1));
-    v[14] = cmul(v[14], cmul(w12, w2));
-    v[15] = cmul(v[15], cmul(w12, w3));
+    // W256^{r k1} from the LDS table (k1 < 16)
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * k1 + r - 1]);
   }
   __syncthreads();  // every thread holds its stage-0 inputs before stage 0 stores
-  run_stages<float, CL_N1, MODE_STRIDED, 0, false, ColGeom::C>(v, lds, c, tau,
-                                                               reinterpret_cast<const cpx<float>*>(a.tw));
+  stage_compute<float, CL_N1, MODE_STRIDED, 0, false, ColGeom::C>(v, tau, nullptr);  // radix 16, no twiddles
+  stage_store_lds<float, CL_N1, MODE_STRIDED, 0, ColGeom::C>(v, lds, c, tau);
+  __syncthreads();
+  stage_load_lds<float, CL_N1, MODE_STRIDED, 1, ColGeom::C>(v, lds, c, tau);
+  // radix-16 stage over sub-transforms of span 16: twiddles W256^{tau r}
+#pragma unroll
+  for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * tau + r - 1]);
+  dft16(v);
   // last stage: point k2 = tau + 16 r of the line; row k1 + 16 k2, 16 lanes on
   // 16 adjacent columns (128 contiguous bytes)
   static_assert(sched_np(CL_N1) == 2 && ColGeom::TPL == 16, "256 = 16 x 16, one butterfly per lane");
@@ -144,7 +136,8 @@ __device__ __forceinline__ void col_stage2(const ColArgs& a, __amdgpu_buffer_rsr
 }
 
 template <int Q, bool INV, bool WL, bool WS>
-__device__ __forceinline__ void col_quarter(const ColArgs& a, unsigned long long o, unsigned grp, cpx<float>* lds) {
+__device__ __forceinline__ void col_quarter(const ColArgs& a, unsigned long long o, unsigned grp, cpx<float>* lds,
+                                            const cpx<float>* tws) {
   // keep the first loads inside this part's code: hoisted above the switch
   // over the part, they were spilled across it
   asm volatile("" ::: "memory");
@@ -190,7 +183,7 @@ __device__ __forceinline__ void col_quarter(const ColArgs& a, unsigned long long
     __builtin_amdgcn_sched_barrier(0);
   }
   __syncthreads();
-  col_stage2<Q, INV, WS>(a, rout, rws, 0, lds);
+  col_stage2<Q, INV, WS>(a, rout, rws, 0, lds, tws);
   __syncthreads();  // the last stage has read the buffer
 #pragma unroll
   for (int i = 0; i < CL_PER; ++i) {
@@ -199,13 +192,16 @@ __device__ __forceinline__ void col_quarter(const ColArgs& a, unsigned long long
     lds[ColGeom::idx(CL_C + c16, n1)] = held[i][1];
   }
   __syncthreads();
-  col_stage2<Q, INV, WS>(a, rout, rws, 1, lds);
+  col_stage2<Q, INV, WS>(a, rout, rws, 1, lds, tws);
 }
 
 // Two workgroups per CU: at most 128 VGPRs, 67.6 KiB LDS each.
 template <bool INV, bool WL, bool WS>
 __global__ void __launch_bounds__(CL_NT, 4) k_fft_col4096(const ColArgs a) {
   __shared__ cpx<float> lds[ColGeom::LDS_ELEMS];
+  __shared__ cpx<float> tws[16 * 15];  // W256^{k r} at [15 k + r - 1]: stage-2 twiddles (read after barriers)
+  if (threadIdx.x < 16 * 15)
+    tws[threadIdx.x] = reinterpret_cast<const cpx<float>*>(a.w256)[(threadIdx.x / 15) * (threadIdx.x % 15 + 1)];
   // XCD-aware mapping as in the 256 x 256 plane pass: block b runs on XCD
   // b % 8; the k-th block of an XCD takes part k % 4 of item (k / 4) * 8 +
   // XCD, so the four parts of a column group are consecutive blocks of one XCD
@@ -217,10 +213,10 @@ __global__ void __launch_bounds__(CL_NT, 4) k_fft_col4096(const ColArgs a) {
   const unsigned grp = (unsigned)(item % a.ngroups);
   const unsigned long long o = item / a.ngroups;
   switch (q) {  // uniform: the part is a compile-time constant inside
-    case 0: col_quarter<0, INV, WL, WS>(a, o, grp, lds); break;
-    case 1: col_quarter<1, INV, WL, WS>(a, o, grp, lds); break;
-    case 2: col_quarter<2, INV, WL, WS>(a, o, grp, lds); break;
-    default: col_quarter<3, INV, WL, WS>(a, o, grp, lds); break;
+    case 0: col_quarter<0, INV, WL, WS>(a, o, grp, lds, tws); break;
+    case 1: col_quarter<1, INV, WL, WS>(a, o, grp, lds, tws); break;
+    case 2: col_quarter<2, INV, WL, WS>(a, o, grp, lds, tws); break;
+    default: col_quarter<3, INV, WL, WS>(a, o, grp, lds, tws); break;
   }
 }
 

@@ -46,6 +46,7 @@ struct ColArgs {
   unsigned long long nitems;     // blocks x column groups
   unsigned ngroups;              // column groups per block (lx / 16)
   const void* tw;                // Stockham twiddles of 256-point lines
+  const void* w256;              // W256^m, m = 0..255
   const void* ptw_lo;            // W4096^t = ptw_hi[t >> ptw_hb] * ptw_lo[t & (2^ptw_hb - 1)]
   const void* ptw_hi;
   unsigned ptw_hb;
