@@ -1168,7 +1168,7 @@ int multi_set_schedule(void* handle, int mode, unsigned grid, unsigned chunk) {
   MultiPlan* mp = static_cast<MultiPlan*>(handle);
   unsigned g = 0, per = 0;
   if (!mp || multi_default_grid(&g, &per, mp->split)) return -1;
-  if (mode < 0 || mode > 2 || (mode == 2 && !mp->plane_ok) || chunk == 0 || grid == 0 || grid > g) {
+  if (mode < 0 || mode > 2 || (mode == 2 && !mp->plane_ok) || grid == 0 || grid > g) {
     set_error("schedule out of range (modes 0, 1%s; grid must be 1..%u resident workgroups)",
               mp->plane_ok ? ", 2" : "", g);
     return -1;
@@ -1219,7 +1219,7 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
   double best = 1e30;
   int bmode = 0;
   unsigned bpc = per, bch = 4;
-  const unsigned chunks[] = {1, 2, 4, 8, 16};
+  const unsigned chunks[] = {1, 2, 4, 8, 16, 0};  // 0: static round-robin, no tickets
   for (int mode = 0; mode <= (mp->plane_ok ? 2 : 1); ++mode)
     for (unsigned pc = 1; pc <= per; ++pc)
       for (unsigned ch : chunks) {

@@ -15,11 +15,19 @@ struct MultiPass {
   int pad;
 };
 
-// Work-queue and barrier counters, zeroed before every launch.
+// Work-queue and barrier counters, zeroed before every launch.  The ticket
+// counters, the barrier counter and the error flag sit on separate 256-byte
+// lines, so that barrier polls do not queue behind ticket atomics.
+constexpr int MULTI_BARRIER_GROUPS = 8;  // workgroups b with b % 8 = g arrive on counter g
+
 struct MultiCtl {
   unsigned head[MULTI_MAX_PHASES];  // next item of each phase
-  unsigned barrier;                 // workgroups that finished a phase
+  unsigned pad0[64 - MULTI_MAX_PHASES];
+  unsigned arrive[MULTI_BARRIER_GROUPS][64];  // per group: arrivals over all phases (own line each)
+  unsigned barrier;                 // groups whose last workgroup finished a phase
+  unsigned pad1[63];
   unsigned err;                     // set when a barrier wait exceeded its bound
+  unsigned pad2[63];
 };
 
 struct MultiItem {

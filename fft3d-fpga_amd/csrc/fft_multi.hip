@@ -80,9 +80,8 @@ constexpr int multi_lds_elems(bool split) {
 }
 
 __global__ void k_multi_reset(MultiCtl* ctl) {
-  const unsigned i = threadIdx.x;
   unsigned* w = reinterpret_cast<unsigned*>(ctl);
-  if (i < sizeof(MultiCtl) / sizeof(unsigned)) w[i] = 0u;
+  for (unsigned i = threadIdx.x; i < sizeof(MultiCtl) / sizeof(unsigned); i += blockDim.x) w[i] = 0u;
 }
 
 // 4 waves per SIMD: the 34.8 KB LDS footprint allows 4 workgroups per CU
@@ -99,9 +98,22 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
     const unsigned gph = phase0 + ph;
     const unsigned base = phase_start[gph];
     const unsigned count = phase_start[gph + 1] - base;
-    for (;;) {
+    if (chunk == 0) {  // static round-robin: tile t to workgroup t mod grid, no tickets
+      for (unsigned t = blockIdx.x; t < count; t += nwg) {
+        __syncthreads();  // the previous tile's LDS reads are done
+        const MultiItem it = items[base + t];
+        const MultiPass* mp = passes + it.pass;
+        dispatch_path<SPLIT>(mp->path, mp->a, it.block, lds);
+      }
+    }
+    for (; chunk != 0;) {
       __syncthreads();  // the previous tile's LDS reads and s_ticket reads are done
-      if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl->head[gph], chunk);
+      // the counter only grows: once a plain read shows the phase drained, no
+      // ticket is taken (a stale read just falls through to the atomic)
+      if (threadIdx.x == 0)
+        s_ticket = __hip_atomic_load(&ctl->head[gph], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= count
+                       ? count
+                       : atomicAdd(&ctl->head[gph], chunk);
       __syncthreads();
       const unsigned t0 = s_ticket;
       if (t0 >= count) break;
@@ -118,8 +130,16 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __syncthreads();
       if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(&ctl->barrier, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned target = (ph + 1) * nwg;
+        // two levels: a workgroup arrives on its group's counter; the last of
+        // its group in this phase arrives on the grid counter, which every
+        // workgroup polls (one read-modify-write per group on the polled line)
+        const unsigned grp = blockIdx.x % MULTI_BARRIER_GROUPS;
+        const unsigned gsize = (nwg - grp + MULTI_BARRIER_GROUPS - 1) / MULTI_BARRIER_GROUPS;
+        const unsigned prev = __hip_atomic_fetch_add(&ctl->arrive[grp][0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev + 1 == (ph + 1) * gsize)
+          __hip_atomic_fetch_add(&ctl->barrier, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned ngroups = nwg < (unsigned)MULTI_BARRIER_GROUPS ? nwg : (unsigned)MULTI_BARRIER_GROUPS;
+        const unsigned target = (ph + 1) * ngroups;
         unsigned polls = 0;
         int abort = 0;
         while (__hip_atomic_load(&ctl->barrier, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {

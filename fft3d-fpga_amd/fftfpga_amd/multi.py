@@ -97,7 +97,8 @@ class MultiPlan:
 
     def set_schedule(self, mode: int, grid: int, chunk: int):
         """mode 0, 1 or 2 (see the module doc); grid (resident workgroups) and
-        chunk (tiles per queue ticket) apply to the persistent modes."""
+        chunk (tiles per queue ticket; 0 = static round-robin, no tickets) apply to
+        the persistent modes."""
         if self._lib.fftamd_multi_set_schedule(self._h, mode, grid, chunk) != 0:
             raise ValueError(last_error())
 

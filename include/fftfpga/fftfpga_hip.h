@@ -115,8 +115,9 @@ int fftamd_multi_tune(fftamd_multi_t plan, void *stream, const char *cache_path,
  * x and y passes as one plane-kernel launch (a whole plane per workgroup, in
  * LDS) and the z passes as one line-pass launch, both over a table of the
  * group's transforms (128^3 groups whose inputs and outputs do not overlap:
- * the y-split pair instead; grid and chunk do not apply).  The tuner considers
- * every available mode. */
+ * the y-split pair instead; grid and chunk do not apply).  chunk: tiles per
+ * queue ticket in modes 0 and 1, or 0 for a static round-robin assignment of
+ * tiles to workgroups (no tickets).  The tuner considers every available mode. */
 int fftamd_multi_schedule(fftamd_multi_t plan, int *mode, unsigned *grid, unsigned *chunk,
                           unsigned *items, unsigned *phases);
 int fftamd_multi_set_schedule(fftamd_multi_t plan, int mode, unsigned grid, unsigned chunk);

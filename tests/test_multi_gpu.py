@@ -39,7 +39,7 @@ MIXED = [(16, 16, 16), (64, 64, 64), (8, 8, 8), (128, 128, 128), (32, 32, 32), (
          (64, 64, 64), (2, 2, 2), (32, 32, 32), (256, 16, 8), (1, 64, 32), (16, 16, 16)]
 
 
-@pytest.mark.parametrize("mode,per_cu,chunk", [(0, None, 4), (1, None, 1), (0, 1, 16), (1, 2, 3)])
+@pytest.mark.parametrize("mode,per_cu,chunk", [(0, None, 4), (1, None, 1), (0, 1, 16), (1, 2, 3), (0, None, 0), (1, 2, 0)])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_mixed_batch_vs_oracle(gpu_lib, orc, mode, per_cu, chunk, inverse):
     import torch
@@ -131,8 +131,8 @@ def _config3_shapes():
     return [(n, n, n) for n in sizes]
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
-def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode):
+@pytest.mark.parametrize("mode,chunk", [(0, None), (1, None), (2, None), (0, 0)])
+def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode, chunk):
     import torch
     from fftfpga_amd import c2c_dev
     from fftfpga_amd.multi import MultiPlan
@@ -142,7 +142,7 @@ def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode):
     d_out = torch.empty_like(d_in)
     plan = MultiPlan(shapes, d_in, d_out)
     sch = plan.schedule()
-    plan.set_schedule(mode, sch["grid"], sch["chunk"])
+    plan.set_schedule(mode, sch["grid"], sch["chunk"] if chunk is None else chunk)
     plan()
     plan.check()
     ref = torch.empty_like(d_in)
