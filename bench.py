@@ -11,8 +11,11 @@ roofline = the dominant kernel: algorithmic bytes of one launch (16 B per point:
          mean launch time from HIP events recorded inside the timed region, as a
          fraction of MI355X's 8 TB/s.  transform_roofline gives the same
          fraction for the whole transform (16 B per point per transform).
-cpu_baseline = the FP64 oracle (a scalar C port, 1 thread) on one forward
-         transform of the same workload, timed on this host.
+cpu_baseline = the reference's CPU path, single-precision FFTW (libfftw3f, or
+         MKL's FFTW3 API), on one forward transform of the same seeded input,
+         threaded over this host's affinity cores with FFTW_ESTIMATE and
+         FFTW_MEASURE plans, plus one thread; cpu_oracle_fp64 times the FP64
+         oracle (a scalar C restatement) on the same transform.
 accuracy = relative L2 error of this run's forward output (first transform)
          against that oracle output, and of the round trip (y / N - x).
 Further objects on the same line (the default workload's run only): plane_pass_path (the same
@@ -22,9 +25,8 @@ steps with 256 x 256 planes as separate x and y passes), torch_fft_rocfft
 (torch.fft on the same data), batched (448 uneven transforms: persistent /
 grouped-plane launch against a per-transform loop), split_512 (split-precision
 512^3), autograd_16x128 (differentiable forward/backward), host_api_pcie (the
-drop-in call on host buffers: copy-in, transform, copy-out times), cpu_fftwf
-(threaded FFTW API on this host), other_workloads (64^3, 4096^2 and 2^22-point
-1D forward transforms).
+drop-in call on host buffers: copy-in, transform, copy-out times),
+other_workloads (64^3, 4096^2 and 2^22-point 1D forward transforms).
 """
 from __future__ import annotations
 
@@ -85,7 +87,7 @@ def parse():
     ap.add_argument("--no-autograd", action="store_true",
                     help="skip the differentiable 16 x 128^3 forward/backward measurement")
     ap.add_argument("--no-cpu-fftw", action="store_true",
-                    help="skip the threaded FFTW-API CPU reference (MKL's fftwf wrappers)")
+                    help="skip the FFTW-API CPU baseline (libfftw3f or MKL's fftwf wrappers)")
     ap.add_argument("--no-other", action="store_true",
                     help="skip the other single-transform configurations (64^3, 4096^2, 2^22 1D)")
     ap.add_argument("--no-host-api", action="store_true",
@@ -93,11 +95,12 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(dims, seed, gpu_forward=None, bitrev=False):
-    """The oracle (FP64 C port, one thread) on one forward transform.  With
-    gpu_forward (the GPU's forward output of the first transform of the same
-    seeded input, complex64), also the relative L2 error of that output against
-    the oracle's (bit-reversed order for the 1D drop-in)."""
+def cpu_oracle(dims, seed, gpu_forward=None, bitrev=False):
+    """The FP64 oracle (a scalar C restatement, one thread) on one forward
+    transform of the same seeded input: its time, and -- with gpu_forward (the
+    GPU's forward output of the first transform, complex64) -- that output's
+    relative L2 error against the oracle's (bit-reversed order for the 1D
+    drop-in)."""
     from oracle import oracle
     x = oracle.uniform(math.prod(dims), seed)
     oracle.fftnd(x, dims)  # warm-up (page-in)
@@ -121,24 +124,39 @@ def cpu_baseline(dims, seed, gpu_forward=None, bitrev=False):
     return out, err
 
 
-def cpu_fftwf(dims, seed):
-    """Threaded single-precision FFTW API on this host's cores: libfftw3f if
-    present, else MKL's FFTW3 wrappers (libmkl_rt).  Forward out-of-place
-    transform of the workload, plan built outside the timed region with
-    FFTW_ESTIMATE; median of 5.  Reported beside the oracle baseline."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        return ""
+
+
+def cpu_fftwf(dims, seed, reps=5):
+    """The reference's CPU path: single-precision FFTW (examples/helper.cpp:119-141
+    verifies with fftwf_plan_many_dft / FFTW_ESTIMATE), here out of place on this
+    host's cores.  libfftw3f(_threads) if present, else MKL's FFTW3-compatible
+    API (libmkl_rt).  Threaded on every core of the affinity mask with plans
+    made under FFTW_ESTIMATE (the reference's flag) and FFTW_MEASURE, plus one
+    thread with FFTW_ESTIMATE; plans are made outside the timed region, one
+    untimed execution, then the median of `reps` executions.  `value` is the
+    fastest of the three, `cores` its thread count."""
     import numpy as np
     from fftfpga_amd import datagen
     lib = label = None
-    for name, lab in (("libfftw3f.so.3", "fftwf"), ("libfftw3f.so", "fftwf"),
-                      ("/opt/conda/lib/libmkl_rt.so", "fftwf API (MKL)"), ("libmkl_rt.so", "fftwf API (MKL)")):
+    for name, lab in (("libfftw3f_threads.so.3", "fftwf"), ("libfftw3f.so.3", "fftwf"),
+                      ("libfftw3f.so", "fftwf"), ("libmkl_rt.so", "fftwf API (MKL)"),
+                      ("/opt/conda/lib/libmkl_rt.so", "fftwf API (MKL)")):
         try:
-            lib, label = ctypes.CDLL(name), lab
-            break
+            cand = ctypes.CDLL(name)
         except OSError:
             continue
+        if hasattr(cand, "fftwf_plan_dft") and hasattr(cand, "fftwf_execute"):
+            lib, label = cand, lab
+            break
     if lib is None:
         return {"available": False, "reason": "no libfftw3f or libmkl_rt on this host"}
-    nthreads = min(16, len(os.sched_getaffinity(0)))
+    ncores = len(os.sched_getaffinity(0))
     vp, ci = ctypes.c_void_p, ctypes.c_int
     plan_dft = lib.fftwf_plan_dft
     plan_dft.restype = vp
@@ -149,33 +167,53 @@ def cpu_fftwf(dims, seed):
     if hasattr(lib, "fftwf_init_threads") and hasattr(lib, "fftwf_plan_with_nthreads"):
         lib.fftwf_plan_with_nthreads.argtypes = [ci]
         threaded = lib.fftwf_init_threads() != 0
+    if hasattr(lib, "fftwf_set_timelimit"):
+        lib.fftwf_set_timelimit.argtypes = [ctypes.c_double]
+        lib.fftwf_set_timelimit(20.0)  # bounds FFTW_MEASURE planning
+    n = math.prod(dims)
+    data = datagen.uniform(n, seed)
+    x = np.empty_like(data)
+    y = np.empty_like(data)
+    nd = (ci * len(dims))(*dims)
+    FFTW_FORWARD, FFTW_MEASURE, FFTW_ESTIMATE = -1, 0, 1 << 6
+    flops = flops_per_transform(dims)
+
+    def run(nthreads, flag, flag_name):
         if threaded:
             lib.fftwf_plan_with_nthreads(nthreads)
-    n = math.prod(dims)
-    x = datagen.uniform(n, seed)
-    y = np.empty_like(x)
-    nd = (ci * len(dims))(*dims)
-    FFTW_FORWARD, FFTW_ESTIMATE = -1, 1 << 6
-    plan = plan_dft(len(dims), nd, x.ctypes.data, y.ctypes.data, FFTW_FORWARD, FFTW_ESTIMATE)
-    if not plan:
-        return {"available": False, "reason": f"{label}: plan creation failed"}
-    lib.fftwf_execute(plan)
-    ts = []
-    for _ in range(5):
         t0 = time.perf_counter()
+        plan = plan_dft(len(dims), nd, x.ctypes.data, y.ctypes.data, FFTW_FORWARD, flag)
+        t_plan = time.perf_counter() - t0
+        if not plan:
+            return {"available": False, "reason": f"{label}: plan creation failed ({flag_name})"}
+        x[:] = data  # FFTW_MEASURE overwrites the arrays while planning
         lib.fftwf_execute(plan)
-        ts.append(time.perf_counter() - t0)
-    lib.fftwf_destroy_plan(plan)
-    t = sorted(ts)[len(ts) // 2]
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
-    return {"value": round(flops_per_transform(dims) / t / 1e9, 3), "unit": "GFLOP/s",
-            "threads": nthreads if threaded else 1, "lib": label, "plan": "FFTW_ESTIMATE",
-            "cpu": model, "sample": f"5 forward {'x'.join(map(str, dims))} transforms, median {t * 1e3:.2f} ms"}
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            lib.fftwf_execute(plan)
+            ts.append(time.perf_counter() - t0)
+        lib.fftwf_destroy_plan(plan)
+        t = sorted(ts)[len(ts) // 2]
+        return {"value": round(flops / t / 1e9, 3), "unit": "GFLOP/s", "ms": round(t * 1e3, 3),
+                "threads": nthreads if threaded else 1, "plan": flag_name, "plan_s": round(t_plan, 3)}
+
+    est = run(ncores, FFTW_ESTIMATE, "FFTW_ESTIMATE")
+    meas = run(ncores, FFTW_MEASURE, "FFTW_MEASURE")
+    one = run(1, FFTW_ESTIMATE, "FFTW_ESTIMATE")
+    # value: the fastest of the three runs (MKL's threading can lose to one
+    # thread on some hosts), with the thread count that reached it
+    best = max((r for r in (est, meas, one) if "value" in r), key=lambda r: r["value"], default=None)
+    if best is None:
+        return {"available": False, "reason": est.get("reason", "fftwf runs failed")}
+    ref = np.fft.fftn(data.reshape(dims).astype(np.complex128)).reshape(-1) if n <= (1 << 24) else None
+    return {"value": best["value"], "unit": "GFLOP/s", "cores": best["threads"], "kind": "fftwf",
+            "lib": label, "plan": best["plan"], "threads_affinity": ncores, "cpu": _cpu_model(),
+            "estimate": est, "measure": meas, "single_thread": one,
+            "rel_l2_vs_numpy_fp64": (float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+                                     if ref is not None else None),
+            "sample": f"forward {'x'.join(map(str, dims))} complex64, out of place, seed {seed}; "
+                      f"median of {reps} executions per plan after one untimed execution"}
 
 
 def split_512(steps):
@@ -544,6 +582,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "traffic_source": (f"profiles/traffic_{args.workload}.json: rocprofv3 FETCH_SIZE + "
+                                        "WRITE_SIZE of this kernel per launch, scaled by the counters of a "
+                                        "plain device copy, from a separate profiled run of this workload "
+                                        "(not measured in this run)") if traffic is not None else None,
                      "avg_launch_us": round(dom_avg_ms * 1e3, 3),
                      "timing": "HIP events around each launch, on its stream, in a second timed "
                                "region of the same K steps",
@@ -608,11 +650,12 @@ def main():
     if default and not args.no_host_api:
         result["host_api_pcie"] = host_api_pcie()
     if not args.no_cpu_baseline:
-        result["cpu_baseline"], err = cpu_baseline(dims, seed, fwd_first, bitrev)
+        result["cpu_oracle_fp64"], err = cpu_oracle(dims, seed, fwd_first, bitrev)
         accuracy["rel_l2_forward_vs_fp64_oracle"] = err
+        if not args.no_cpu_fftw:
+            # the reference's own CPU path (FFTW single precision) on this host
+            result["cpu_baseline"] = cpu_fftwf(dims, seed)
     result["accuracy"] = accuracy
-    if not args.no_cpu_fftw:
-        result["cpu_fftwf"] = cpu_fftwf(dims, seed)
     result["cpu_reference_emulation"] = {
         "available": False,
         "reason": "the reference's CPU run (fft --num=64 --dim=3 --emulate) needs the Intel FPGA "

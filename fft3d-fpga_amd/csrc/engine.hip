@@ -29,8 +29,14 @@
 #include "fft_pass.h"
 #include "fft_plane.h"
 #include "fft_schedule.h"
+#include "launch.h"
 
 namespace fftamd {
+
+bool& resolve_only_flag() {
+  static thread_local bool on = false;
+  return on;
+}
 
 // ---------------------------------------------------------------------------
 // error reporting
@@ -101,6 +107,10 @@ struct State {
   std::map<std::pair<int, int>, void*> tw_stockham;  // (prec, L)
   std::map<std::pair<int, long long>, std::pair<void*, void*>> tw_fourstep;  // (prec, M)
   std::map<long long, void*> tw_direct;  // FP32 W_M^t, t = 0..M-1
+  // bumped whenever tables and plans are released (fpga_final, re-init):
+  // multi-transform plans and graphs made before hold freed table pointers
+  unsigned long long gen = 1;
+  struct Pipe* pipe = nullptr;  // host batch pipeline buffers (grow-only)
 };
 
 static State& S() {
@@ -312,9 +322,10 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   const bool bitrev = (flags & FLAG_BITREV_1D) && rank == 1;
   std::vector<PassDesc> ps;
 
-  // n x n x n blocks, n = 256 or 128 (the three fastest axes): two passes with
-  // the y axis split (n/4) x 4 between them -- x and y over b (out of place),
-  // then y over a and z (in place)
+  // n x n x n blocks, n = 512, 256 or 128 (the three fastest axes): two
+  // passes with the y axis split between them (64 x 8 for 512, (n/4) x 4
+  // otherwise) -- x and the larger y stage (out of place), then the smaller y
+  // stage and z (in place)
   int top = rank - 1;
   const long long CN = rank >= 3 ? (long long)dims[rank - 1] : 0;
   if (rank >= 3 && prec == PREC_F32 && !plan.split && !(flags & (FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT)) &&
@@ -555,6 +566,23 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   return 0;
 }
 
+static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipStream_t stream,
+                    const float* w_load = nullptr, const float* w_store = nullptr);
+
+// Everything a plan's calls need before the first of them: its twiddle tables
+// and the code objects of its kernels (both directions), made by running the
+// plan once with the launchers in resolve-only mode.
+static int plan_prepare(Plan& plan) {
+  struct Resolve {
+    Resolve() { resolve_only_flag() = true; }
+    ~Resolve() { resolve_only_flag() = false; }
+  } guard;
+  static char dummy[64];
+  for (int inv = 0; inv < 2; ++inv)
+    if (run_plan(plan, dummy, dummy, inv != 0, nullptr)) return -1;
+  return 0;
+}
+
 static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
                     Plan** out) {
   State& st = S();
@@ -569,13 +597,19 @@ static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, 
       HIP_TRY(hipMalloc(&p.tmp, bytes));
     }
     it = st.plans.emplace(key, std::move(p)).first;
+    if (plan_prepare(it->second)) {
+      if (it->second.tmp) (void)hipFree(it->second.tmp);
+      st.plans.erase(it);
+      return -1;
+    }
   }
   *out = &it->second;
   return 0;
 }
 
 static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipStream_t stream,
-                    const float* w_load = nullptr, const float* w_store = nullptr) {
+                    const float* w_load, const float* w_store) {
+  const bool resolve = resolve_only_flag();
   if ((w_load || w_store) && plan.passes.empty()) {
     set_error("weighted transform of a single point is not supported");
     return -1;
@@ -601,7 +635,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       xa.w_mask = (unsigned long long)plan.per - 1ULL;
       xa.inverse = inverse ? 1 : 0;
       Prof& pf = P();
-      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      const bool prof = pf.on && !resolve && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
       const int rc = (LY == 256 && LX == 256) ? launch_fft_xy256(xa, stream)
                                                : launch_fft_plane(LY, LX, xa, stream, plan.split);
@@ -626,8 +660,10 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       ca.out = ptr(p.dst);
       ca.n = 1 << p.lg;
       ca.nplanes = A ? p.nouter : p.nouter * (unsigned long long)ca.n;
-      if (get_stockham(PREC_F32, ca.n, &ca.tw)) return -1;
+      // 512: pass A's columns are 64-point lines, pass B's z lines 4 x 128
+      if (get_stockham(PREC_F32, ca.n == 512 ? (A ? 64 : 128) : ca.n, &ca.tw)) return -1;
       if (get_direct(256, &ca.w256)) return -1;
+      if (ca.n == 512 && get_direct(512, &ca.w512)) return -1;
       // pass A is always the first pass of its plan; pass B takes the store
       // weights when it is the last
       ca.w_load = A && pi == 0 ? w_load : nullptr;
@@ -635,7 +671,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       ca.w_mask = (unsigned long long)plan.per - 1ULL;
       ca.inverse = inverse ? 1 : 0;
       Prof& pf = P();
-      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      const bool prof = pf.on && !resolve && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
       const int rc = A ? launch_fft_cube_a(ca, stream) : launch_fft_cube_b(ca, stream);
       if (prof) {
@@ -669,7 +705,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       ca.w_mask = (unsigned long long)plan.per - 1ULL;
       ca.inverse = inverse ? 1 : 0;
       Prof& pf = P();
-      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      const bool prof = pf.on && !resolve && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
       const int rc = launch_fft_col4096(ca, stream);
       if (prof) {
@@ -723,7 +759,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       const unsigned long long no = std::min(outer_per_launch, p.nouter - o0);
       a.outer0 = o0;
       Prof& pf = P();
-      const bool prof = pf.on && pf.used + 2 <= pf.pool.size();
+      const bool prof = pf.on && !resolve && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
       int rc = launch_fft_pass(plan.prec, p.lg, p.mode, p.epi, a, per_outer * no, stream, split);
       if (prof) {
@@ -744,7 +780,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
     }
     cur = p.dst;
   }
-  if (plan.final_copy) {
+  if (plan.final_copy && !resolve) {
     const void* src = plan.passes.empty() ? din : ptr(cur);
     if (src != dout) HIP_TRY(hipMemcpyAsync(dout, src, plan.total * esz, hipMemcpyDeviceToDevice, stream));
   }
@@ -754,6 +790,8 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
 
 static bool check_dims(int rank, const size_t* dims, size_t batch);
 static double ms_between(hipEvent_t a, hipEvent_t b);
+static int check_dev_buffers(int rank, const size_t* dims, size_t batch, const void* din, const void* dout,
+                             size_t esz);
 
 // ---------------------------------------------------------------------------
 // multi-transform plans: transforms of uneven sizes in one persistent launch
@@ -779,9 +817,10 @@ struct MultiPlan {
   // 0: one launch, grid barriers between phases; 1: one launch per phase;
   // 2: x and y of each shape group as one plane-kernel launch, then its z
   // passes as one line-pass launch (transform tables; no persistent kernel)
-  int mode = 0;
+  int mode = 1;
   bool split = false;
-  bool plane_ok = false;  // mode 2 available (every transform 3D, planes up to 128 x 128)
+  bool plane_ok = false;
+  unsigned long long gen = 0;  // library generation the plan's tables belong to  // mode 2 available (every transform 3D, planes up to 128 x 128)
   std::vector<PlaneGroup> groups;
   unsigned long long* d_tab = nullptr;    // (input, output) offset per transform, by group
   unsigned long long* d_tab_z = nullptr;  // (output, output): the z passes run in place
@@ -854,6 +893,10 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
       }
     Plan p;
     const bool aliased = (const char*)din + 8 * in_off[i] == (const char*)dout + 8 * out_off[i];
+    if (!aliased && check_dev_buffers(3, d, 1, (const char*)din + 8 * in_off[i], (const char*)dout + 8 * out_off[i], 8)) {
+      set_error("transform %u: input and output overlap without being the same range", i);
+      return -1;
+    }
     // line passes only: the persistent kernel's tiles are 256-thread line tiles
     if (build_plan(3, d, 1, FLAG_NO_PLANE_FUSION, PREC_F32, aliased, p)) return -1;
     for (const PassDesc& q : p.passes)
@@ -991,6 +1034,11 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
   }
   unsigned per = 0;
   if (multi_default_grid(&mp->grid, &per, mp->split)) return -1;
+  // untuned: the grouped plane launches where they apply, else one launch per
+  // phase; neither waits on a grid barrier, which needs every workgroup of
+  // the grid resident at once
+  mp->mode = mp->plane_ok ? 2 : 1;
+  mp->gen = st.gen;
   *handle = mp.release();
   return 0;
 }
@@ -1087,12 +1135,26 @@ static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stre
   return 0;
 }
 
+// A plan made before the library released its tables (fpga_final or a new
+// fpga_initialize) holds pointers to freed twiddle tables: refuse it.
+static bool multi_current(const MultiPlan* mp) {
+  const State& st = S();
+  if (!st.ready || mp->gen != st.gen) {
+    set_error("multi-transform plan or graph predates fpga_final / fpga_initialize; create it again");
+    return false;
+  }
+  return true;
+}
+
 int multi_exec(void* handle, void* stream) {
   MultiPlan* mp = static_cast<MultiPlan*>(handle);
   if (!mp) {
     set_error("null multi-transform handle");
     return -1;
   }
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (!multi_current(mp)) return -1;
   if (mp->nphase == 0) return 0;
   hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
   if (mp->mode == 2) {
@@ -1189,6 +1251,7 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
   if (!mp) return -1;
   State& st = S();
   std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (!multi_current(mp)) return -1;
   unsigned gmax = 0, per = 0;
   if (multi_default_grid(&gmax, &per, mp->split)) return -1;
   const unsigned ncu = gmax / per;
@@ -1264,6 +1327,7 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
 struct Graph {
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
+  unsigned long long gen = 0;  // library generation of the captured plans' tables
   ~Graph() {
     if (x) (void)hipGraphExecDestroy(x);
     if (g) (void)hipGraphDestroy(g);
@@ -1284,8 +1348,10 @@ int graph_capture_multi(void** out, void** handles, int n, void* stream) {
       set_error("null multi-transform handle");
       return -1;
     }
+    if (!multi_current(static_cast<MultiPlan*>(handles[i]))) return -1;
     if (multi_prepare_fork(static_cast<MultiPlan*>(handles[i]))) return -1;  // no creation under capture
   }
+  gr->gen = st.gen;
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   for (int i = 0; i < n && !rc; ++i) rc = multi_exec(handles[i], s);
@@ -1308,6 +1374,14 @@ int graph_capture_multi(void** out, void** handles, int n, void* stream) {
 int graph_launch(void* graph, void* stream) {
   Graph* g = static_cast<Graph*>(graph);
   if (!g) return -1;
+  {
+    State& st = S();
+    std::lock_guard<std::recursive_mutex> lk(st.mu);
+    if (!st.ready || g->gen != st.gen) {
+      set_error("graph predates fpga_final / fpga_initialize; capture it again");
+      return -1;
+    }
+  }
   HIP_TRY(hipGraphLaunch(g->x, reinterpret_cast<hipStream_t>(stream)));
   return 0;
 }
@@ -1329,8 +1403,13 @@ bool runtime_can_map_host(int dev) {
   return v != 0;
 }
 
+static void pipe_release(Pipe* p);
+
 static void release_locked() {
   State& st = S();
+  ++st.gen;  // multi-transform plans and graphs made so far are now stale
+  pipe_release(st.pipe);
+  st.pipe = nullptr;
   for (auto& kv : st.plans)
     if (kv.second.tmp) (void)hipFree(kv.second.tmp);
   st.plans.clear();
@@ -1489,13 +1568,13 @@ int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, in
                        (long long)d.nouter, d.M, d.cshift, d.src, d.dst,
                        d.mode == MODE_PLANE ? (d.K == 256 ? 64 : (long long)d.K)
                        : d.mode == MODE_COL4096 ? 16
-                       : d.mode == MODE_CUBE_A ? (1LL << d.lg) / 4   // rows per workgroup
-                       : d.mode == MODE_CUBE_B ? (d.lg == 8 ? 16 : 32)   // columns per workgroup (x 4 rows x n planes)
+                       : d.mode == MODE_CUBE_A ? (d.lg == 9 ? 64 : (1LL << d.lg) / 4)   // rows per workgroup
+                       : d.mode == MODE_CUBE_B ? (d.lg == 9 ? 8 : d.lg == 8 ? 16 : 32)   // columns per workgroup (x 8 or 4 rows x n planes)
                        : geom_cols_for(1 << d.lg, d.mode, sp(d)),
                        d.mode == MODE_PLANE ? plane_threads(d)
                        : d.mode == MODE_COL4096 ? 512
-                       : d.mode == MODE_CUBE_A ? (d.lg == 8 ? 512 : 256)
-                       : d.mode == MODE_CUBE_B ? 512
+                       : d.mode == MODE_CUBE_A ? (d.lg == 9 ? 1024 : d.lg == 8 ? 512 : 256)
+                       : d.mode == MODE_CUBE_B ? (d.lg == 9 ? 1024 : 512)
                        : geom_threads_for(1 << d.lg, d.mode, sp(d))};
     for (int k = 0; k < NF; ++k) f[k] = v[k];
   }
@@ -1624,6 +1703,7 @@ struct Pipe {
   void* pin_out[2] = {nullptr, nullptr};
   void* d_in[2] = {nullptr, nullptr};
   void* d_out[2] = {nullptr, nullptr};
+  size_t cap = 0;  // bytes of each chunk buffer
   ~Pipe() {
     for (auto st : s)
       if (st) (void)hipStreamSynchronize(st);
@@ -1641,6 +1721,8 @@ struct Pipe {
   }
 };
 
+static void pipe_release(Pipe* p) { delete p; }
+
 int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* hin, void* hout,
                        bool inverse, Timing* t) {
   using clk = std::chrono::steady_clock;
@@ -1652,7 +1734,6 @@ int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* h
   }
   if (!check_dims(rank, dims, count)) return -1;
   HIP_TRY(hipSetDevice(st.dev));
-  auto t0 = clk::now();
   size_t per = 1;
   for (int d = 0; d < rank; ++d) per *= dims[d];
   const size_t tbytes = per * 8;
@@ -1660,17 +1741,45 @@ int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* h
   const size_t chunk = std::max<size_t>(1, std::min<size_t>(count, (64u << 20) / std::max<size_t>(tbytes, 1)));
   const size_t nchunks = (count + chunk - 1) / chunk;
   const size_t cbytes = chunk * tbytes;
-  Pipe pp;
-  for (auto& s : pp.s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  for (int i = 0; i < 2; ++i) {
-    HIP_TRY(hipEventCreateWithFlags(&pp.h2d[i], hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&pp.ex[i], hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&pp.d2h[i], hipEventDisableTiming));
-    HIP_TRY(hipHostMalloc(&pp.pin_in[i], cbytes, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc(&pp.pin_out[i], cbytes, hipHostMallocDefault));
-    HIP_TRY(hipMalloc(&pp.d_in[i], cbytes));
-    HIP_TRY(hipMalloc(&pp.d_out[i], cbytes));
+  // streams, events and the pinned and device chunk buffers live across calls
+  // (released by fpga_final); the buffers only grow
+  if (!st.pipe) {
+    auto np = std::make_unique<Pipe>();
+    for (auto& s : np->s) HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+      HIP_TRY(hipEventCreateWithFlags(&np->h2d[i], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&np->ex[i], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&np->d2h[i], hipEventDisableTiming));
+    }
+    st.pipe = np.release();
   }
+  Pipe& pp = *st.pipe;
+  if (pp.cap < cbytes) {
+    for (int i = 0; i < 2; ++i) {
+      if (pp.pin_in[i]) HIP_TRY(hipHostFree(pp.pin_in[i]));
+      if (pp.pin_out[i]) HIP_TRY(hipHostFree(pp.pin_out[i]));
+      if (pp.d_in[i]) HIP_TRY(hipFree(pp.d_in[i]));
+      if (pp.d_out[i]) HIP_TRY(hipFree(pp.d_out[i]));
+      pp.pin_in[i] = pp.pin_out[i] = pp.d_in[i] = pp.d_out[i] = nullptr;
+    }
+    pp.cap = 0;
+    for (int i = 0; i < 2; ++i) {
+      HIP_TRY(hipHostMalloc(&pp.pin_in[i], cbytes, hipHostMallocDefault));
+      HIP_TRY(hipHostMalloc(&pp.pin_out[i], cbytes, hipHostMallocDefault));
+      HIP_TRY(hipMalloc(&pp.d_in[i], cbytes));
+      HIP_TRY(hipMalloc(&pp.d_out[i], cbytes));
+    }
+    pp.cap = cbytes;
+  }
+  // plans (and their tables and code objects) for the full chunk and a
+  // partial last chunk, before the clock starts -- the reference also creates
+  // its buffers before timing a batch (api/src/fft3d.c:431-489)
+  Plan* plan_full = nullptr;
+  Plan* plan_last = nullptr;
+  const size_t last_n = count - (nchunks - 1) * chunk;
+  if (get_plan(rank, dims, chunk, 0u, PREC_F32, false, &plan_full)) return -1;
+  if (get_plan(rank, dims, last_n, 0u, PREC_F32, false, &plan_last)) return -1;
+  auto t0 = clk::now();
   const char* src = static_cast<const char*>(hin);
   char* dst = static_cast<char*>(hout);
   auto nb = [&](size_t i) { return std::min(chunk, count - i * chunk) * tbytes; };
@@ -1691,8 +1800,7 @@ int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* h
     HIP_TRY(hipEventRecord(pp.h2d[slot], pp.s[0]));
     HIP_TRY(hipStreamWaitEvent(pp.s[1], pp.h2d[slot], 0));
     if (i >= 2) HIP_TRY(hipStreamWaitEvent(pp.s[1], pp.d2h[slot], 0));  // device output slot free
-    Plan* plan = nullptr;
-    if (get_plan(rank, dims, ntr, 0u, PREC_F32, false, &plan)) return -1;
+    Plan* plan = ntr == chunk ? plan_full : plan_last;
     if (run_plan(*plan, pp.d_in[slot], pp.d_out[slot], inverse, pp.s[1])) return -1;
     HIP_TRY(hipEventRecord(pp.ex[slot], pp.s[1]));
     if (i >= 2 && drain(i - 2)) return -1;  // pinned output slot free
@@ -1710,16 +1818,31 @@ int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* h
   return 0;
 }
 
-int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din, void* dout,
-                     bool inverse, const float* w_load, const float* w_store, void* stream) {
-  State& st = S();
-  std::lock_guard<std::recursive_mutex> lk(st.mu);
-  if (runtime_ensure_device()) return -1;
-  if (!check_dims(rank, dims, batch)) return -1;
+// Device buffers of one call: null pointers and input/output ranges that
+// overlap without being the same buffer are refused (the first pass of a plan
+// may run out of place, so a partial overlap would corrupt the input).
+static int check_dev_buffers(int rank, const size_t* dims, size_t batch, const void* din, const void* dout,
+                             size_t esz) {
   if (!din || !dout) {
     set_error("null device pointer");
     return -1;
   }
+  size_t bytes = batch * esz;
+  for (int d = 0; d < rank; ++d) bytes *= dims[d];
+  const uintptr_t a = reinterpret_cast<uintptr_t>(din), b = reinterpret_cast<uintptr_t>(dout);
+  if (a != b && a < b + bytes && b < a + bytes) {
+    set_error("input and output buffers overlap without being the same buffer (in place needs d_in == d_out)");
+    return -1;
+  }
+  return 0;
+}
+
+int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din, void* dout,
+                     bool inverse, const float* w_load, const float* w_store, void* stream) {
+  if (!check_dims(rank, dims, batch) || check_dev_buffers(rank, dims, batch, din, dout, 8)) return -1;
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (runtime_ensure_device()) return -1;
   Plan* plan = nullptr;
   if (get_plan(rank, dims, batch, 0u, PREC_F32, din == dout, &plan)) return -1;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
@@ -1728,14 +1851,12 @@ int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din
 
 int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
             unsigned flags, int prec, void* stream) {
+  if (!check_dims(rank, dims, batch) ||
+      check_dev_buffers(rank, dims, batch, din, dout, prec == PREC_F32 ? 8 : 16))
+    return -1;
   State& st = S();
   std::lock_guard<std::recursive_mutex> lk(st.mu);
   if (runtime_ensure_device()) return -1;
-  if (!check_dims(rank, dims, batch)) return -1;
-  if (!din || !dout) {
-    set_error("null device pointer");
-    return -1;
-  }
   Plan* plan = nullptr;
   const bool aliased = din == dout;
   if (get_plan(rank, dims, batch, flags, prec, aliased, &plan)) return -1;

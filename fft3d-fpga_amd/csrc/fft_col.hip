@@ -22,6 +22,8 @@
 // VGPR offset per lane and scalar row offsets.  Out of place only.
 #include <hip/hip_runtime.h>
 
+#include "launch.h"
+
 #include "fft_pass_dev.h"
 #include "fft_plane.h"
 
@@ -222,7 +224,7 @@ __global__ void __launch_bounds__(CL_NT, 4) k_fft_col4096(const ColArgs a) {
 
 template <bool INV, bool WL, bool WS>
 static void launch_colw(const ColArgs& a, hipStream_t s, unsigned blocks) {
-  hipLaunchKernelGGL((k_fft_col4096<INV, WL, WS>), dim3(blocks), dim3(CL_NT), 0, s, a);
+  FFTAMD_LAUNCH((k_fft_col4096<INV, WL, WS>), dim3(blocks), dim3(CL_NT), 0, s, a);
 }
 
 template <bool INV>

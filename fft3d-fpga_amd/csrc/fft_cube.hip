@@ -29,6 +29,8 @@
 // B's stores.
 #include <hip/hip_runtime.h>
 
+#include "launch.h"
+
 #include "fft_pass_dev.h"
 #include "fft_plane.h"
 
@@ -402,6 +404,7 @@ static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsign
 }
 
 int launch_fft_cube_a(const CubeArgs& a, void* stream) {
+  if (a.n == 512) return launch_fft_c512_a(a, stream);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
   if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || !cube_grid(a.nplanes, 4, &blocks) || a.w_store ||
@@ -410,26 +413,27 @@ int launch_fft_cube_a(const CubeArgs& a, void* stream) {
   if (a.n == C1_L) {
     const dim3 g((unsigned)blocks), t(C1_NT);
     if (a.inverse) {
-      if (a.w_load) hipLaunchKernelGGL((k_fft_cube128_a<true, true>), g, t, 0, s, a);
-      else hipLaunchKernelGGL((k_fft_cube128_a<true, false>), g, t, 0, s, a);
+      if (a.w_load) FFTAMD_LAUNCH((k_fft_cube128_a<true, true>), g, t, 0, s, a);
+      else FFTAMD_LAUNCH((k_fft_cube128_a<true, false>), g, t, 0, s, a);
     } else {
-      if (a.w_load) hipLaunchKernelGGL((k_fft_cube128_a<false, true>), g, t, 0, s, a);
-      else hipLaunchKernelGGL((k_fft_cube128_a<false, false>), g, t, 0, s, a);
+      if (a.w_load) FFTAMD_LAUNCH((k_fft_cube128_a<false, true>), g, t, 0, s, a);
+      else FFTAMD_LAUNCH((k_fft_cube128_a<false, false>), g, t, 0, s, a);
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   const dim3 g((unsigned)blocks), t(CB_NT);
   if (a.inverse) {
-    if (a.w_load) hipLaunchKernelGGL((k_fft_cube_a<true, true>), g, t, 0, s, a);
-    else hipLaunchKernelGGL((k_fft_cube_a<true, false>), g, t, 0, s, a);
+    if (a.w_load) FFTAMD_LAUNCH((k_fft_cube_a<true, true>), g, t, 0, s, a);
+    else FFTAMD_LAUNCH((k_fft_cube_a<true, false>), g, t, 0, s, a);
   } else {
-    if (a.w_load) hipLaunchKernelGGL((k_fft_cube_a<false, true>), g, t, 0, s, a);
-    else hipLaunchKernelGGL((k_fft_cube_a<false, false>), g, t, 0, s, a);
+    if (a.w_load) FFTAMD_LAUNCH((k_fft_cube_a<false, true>), g, t, 0, s, a);
+    else FFTAMD_LAUNCH((k_fft_cube_a<false, false>), g, t, 0, s, a);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int launch_fft_cube_b(const CubeArgs& a, void* stream) {
+  if (a.n == 512) return launch_fft_c512_b(a, stream);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
   if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || a.nplanes % a.n || a.w_load ||
@@ -439,22 +443,22 @@ int launch_fft_cube_b(const CubeArgs& a, void* stream) {
     if (!cube_grid(a.nplanes / C1_L, 128, &blocks)) return -1;
     const dim3 g((unsigned)blocks), t(C1B_NT);
     if (a.inverse) {
-      if (a.w_store) hipLaunchKernelGGL((k_fft_cube128_b<true, true>), g, t, 0, s, a);
-      else hipLaunchKernelGGL((k_fft_cube128_b<true, false>), g, t, 0, s, a);
+      if (a.w_store) FFTAMD_LAUNCH((k_fft_cube128_b<true, true>), g, t, 0, s, a);
+      else FFTAMD_LAUNCH((k_fft_cube128_b<true, false>), g, t, 0, s, a);
     } else {
-      if (a.w_store) hipLaunchKernelGGL((k_fft_cube128_b<false, true>), g, t, 0, s, a);
-      else hipLaunchKernelGGL((k_fft_cube128_b<false, false>), g, t, 0, s, a);
+      if (a.w_store) FFTAMD_LAUNCH((k_fft_cube128_b<false, true>), g, t, 0, s, a);
+      else FFTAMD_LAUNCH((k_fft_cube128_b<false, false>), g, t, 0, s, a);
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   if (!cube_grid(a.nplanes / CB_L, 1024, &blocks)) return -1;
   const dim3 g((unsigned)blocks), t(CB_NT);
   if (a.inverse) {
-    if (a.w_store) hipLaunchKernelGGL((k_fft_cube_b<true, true>), g, t, 0, s, a);
-    else hipLaunchKernelGGL((k_fft_cube_b<true, false>), g, t, 0, s, a);
+    if (a.w_store) FFTAMD_LAUNCH((k_fft_cube_b<true, true>), g, t, 0, s, a);
+    else FFTAMD_LAUNCH((k_fft_cube_b<true, false>), g, t, 0, s, a);
   } else {
-    if (a.w_store) hipLaunchKernelGGL((k_fft_cube_b<false, true>), g, t, 0, s, a);
-    else hipLaunchKernelGGL((k_fft_cube_b<false, false>), g, t, 0, s, a);
+    if (a.w_store) FFTAMD_LAUNCH((k_fft_cube_b<false, true>), g, t, 0, s, a);
+    else FFTAMD_LAUNCH((k_fft_cube_b<false, false>), g, t, 0, s, a);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

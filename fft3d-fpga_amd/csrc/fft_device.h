@@ -126,6 +126,45 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   for (int i = 0; i < 16; ++i) v[i] = t[i];
 }
 
+// 32-point DFT as 2 x 16: X[k + 16 f] = E[k] + (-1)^f W32^k O[k] with E, O the
+// 16-point DFTs of the even and odd points.  W32^k = cos(pi k / 16) - i
+// sin(pi k / 16); the odd k use cos and sin of pi/16 and 3 pi/16.
+template <typename R>
+__device__ __forceinline__ void dft32(cpx<R>* v) {
+  cpx<R> e[16], o[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    e[i] = v[2 * i];
+    o[i] = v[2 * i + 1];
+  }
+  dft16(e);
+  dft16(o);
+  constexpr R c1 = R(0.98078528040323044912618223613424268L);  // cos(pi/16)
+  constexpr R s1 = R(0.19509032201612826784828486847702224L);  // sin(pi/16)
+  constexpr R c3 = R(0.83146961230254523707878837761790576L);  // cos(3 pi/16)
+  constexpr R s3 = R(0.55557023301960222474283081394853287L);  // sin(3 pi/16)
+  o[1] = cmul(o[1], cpx<R>{c1, -s1});
+  o[2] = cmul(o[2], cpx<R>{K<R>::C16, -K<R>::S16});
+  o[3] = cmul(o[3], cpx<R>{c3, -s3});
+  o[4] = mul_w8_1(o[4]);
+  o[5] = cmul(o[5], cpx<R>{s3, -c3});
+  o[6] = cmul(o[6], cpx<R>{K<R>::S16, -K<R>::C16});
+  o[7] = cmul(o[7], cpx<R>{s1, -c1});
+  o[8] = cmnegi(o[8]);
+  o[9] = cmul(o[9], cpx<R>{-s1, -c1});
+  o[10] = cmul(o[10], cpx<R>{-K<R>::S16, -K<R>::C16});
+  o[11] = cmul(o[11], cpx<R>{-s3, -c3});
+  o[12] = mul_w8_3(o[12]);
+  o[13] = cmul(o[13], cpx<R>{-c3, -s3});
+  o[14] = cmul(o[14], cpx<R>{-K<R>::C16, -K<R>::S16});
+  o[15] = cmul(o[15], cpx<R>{-c1, -s1});
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    v[k] = cadd(e[k], o[k]);
+    v[k + 16] = csub(e[k], o[k]);
+  }
+}
+
 template <typename R, int RAD>
 __device__ __forceinline__ void dft(cpx<R>* v) {
   if constexpr (RAD == 2) dft2(v);

@@ -14,6 +14,8 @@
 // error instead of hanging.  The host sizes the grid to what is resident.
 #include <hip/hip_runtime.h>
 
+#include "launch.h"
+
 #include "fft_multi.h"
 #include "fft_pass_dev.h"
 
@@ -160,7 +162,7 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
 }
 
 int reset_multi_ctl(MultiCtl* ctl, void* stream) {
-  hipLaunchKernelGGL(k_multi_reset, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), ctl);
+  FFTAMD_LAUNCH(k_multi_reset, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), ctl);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -168,12 +170,12 @@ int launch_multi(const MultiPass* passes, const MultiItem* items, const unsigned
                  unsigned phase0, unsigned nphase, MultiCtl* ctl, unsigned grid, unsigned chunk,
                  unsigned spin_limit, void* stream, bool split) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_multi_reset, dim3(1), dim3(64), 0, s, ctl);
+  FFTAMD_LAUNCH(k_multi_reset, dim3(1), dim3(64), 0, s, ctl);
   if (split)
-    hipLaunchKernelGGL(k_fft_multi<true>, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items, phase_start,
+    FFTAMD_LAUNCH(k_fft_multi<true>, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items, phase_start,
                        phase0, nphase, ctl, chunk, spin_limit);
   else
-    hipLaunchKernelGGL(k_fft_multi<false>, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items,
+    FFTAMD_LAUNCH(k_fft_multi<false>, dim3(grid), dim3(MULTI_THREADS), 0, s, passes, items,
                        phase_start, phase0, nphase, ctl, chunk, spin_limit);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

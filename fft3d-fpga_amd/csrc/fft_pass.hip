@@ -13,6 +13,8 @@
 // launch table over (precision, line length, mode, epilogue).
 #include <hip/hip_runtime.h>
 
+#include "launch.h"
+
 #include "fft_pass_dev.h"
 
 namespace fftamd {
@@ -35,9 +37,9 @@ static void launch_one(const PassArgs& a, unsigned long long blocks, hipStream_t
   // one kernel per direction: the direction swaps become register renames
   constexpr int NT = geom_threads_for(L, MODE, SPLIT);
   if (a.inverse)
-    hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI, SPLIT, 1>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
+    FFTAMD_LAUNCH((k_fft_pass<R, L, MODE, EPI, SPLIT, 1>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
   else
-    hipLaunchKernelGGL((k_fft_pass<R, L, MODE, EPI, SPLIT, 0>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
+    FFTAMD_LAUNCH((k_fft_pass<R, L, MODE, EPI, SPLIT, 0>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
 }
 
 template <typename R, int MODE, int EPI, int LG, bool SPLIT>

@@ -61,19 +61,22 @@ struct ColArgs {
 int launch_fft_col4096(const ColArgs& a, void* stream);
 bool col4096_supported(long long lx, unsigned long long nouter);
 
-// Arrays of n x n x n blocks (n = 256 or 128) in two passes with the y axis
-// split (n/4) x 4 (fft_cube.hip).  Pass A: x and the (n/4)-point y stage of
-// every plane, out of place only.  Pass B: the 4-point y stage and z of every block, in place
-// allowed.  Pass A takes load weights only, pass B store weights only (see
-// PassArgs).  0 on success, -1 for an unsupported shape or grid, -2 on a
-// launch error.
+// Arrays of n x n x n blocks (n = 512, 256 or 128) in two passes with the y
+// axis split between them: (n/4) x 4 for n = 256 and 128 (fft_cube.hip),
+// 64 x 8 for n = 512 (fft_cube512.hip).  Pass A: x and the larger y stage of
+// every plane, out of place only.  Pass B: the smaller y stage and z of every
+// block, in place allowed.  Pass A takes load weights only, pass B store
+// weights only (see PassArgs).  0 on success, -1 for an unsupported shape or
+// grid, -2 on a launch error.
 struct CubeArgs {
   const void* in;
   void* out;
-  int n;                         // block side: 256 or 128
+  int n;                         // block side: 512, 256 or 128
   unsigned long long nplanes;    // n x n planes in the array (a multiple of n for pass B)
-  const void* tw;                // Stockham twiddles of n-point lines
+  const void* tw;                // Stockham twiddles of n-point lines (n = 512: of the
+                                 // 64-point lines of pass A, the 128-point lines of pass B)
   const void* w256;              // W256^m, m = 0..255
+  const void* w512;              // W512^m, m = 0..511 (n = 512)
   const float* w_load;
   const float* w_store;
   unsigned long long w_mask;
@@ -85,5 +88,7 @@ struct CubeArgs {
 };
 int launch_fft_cube_a(const CubeArgs& a, void* stream);
 int launch_fft_cube_b(const CubeArgs& a, void* stream);
+int launch_fft_c512_a(const CubeArgs& a, void* stream);  // n = 512 (fft_cube512.hip)
+int launch_fft_c512_b(const CubeArgs& a, void* stream);
 
 }  // namespace fftamd

@@ -15,6 +15,8 @@
 // k_fft_plane_halves below (two workgroups per CU).
 #include <hip/hip_runtime.h>
 
+#include "launch.h"
+
 #include "fft_pass_dev.h"
 #include "fft_plane.h"
 
@@ -202,11 +204,11 @@ template <int L, bool INV, bool SPLIT>
 static void launch_halves(const PlaneArgs& a, hipStream_t s) {
   const dim3 grid((unsigned)a.nplanes), block(HalfGeom<L, SPLIT>::NT);
   if (a.w_load) {
-    if (a.w_store) hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, true, false>), grid, block, 0, s, a);
+    if (a.w_store) FFTAMD_LAUNCH((k_fft_plane_halves<L, INV, SPLIT, true, true>), grid, block, 0, s, a);
+    else FFTAMD_LAUNCH((k_fft_plane_halves<L, INV, SPLIT, true, false>), grid, block, 0, s, a);
   } else {
-    if (a.w_store) hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_fft_plane_halves<L, INV, SPLIT, false, false>), grid, block, 0, s, a);
+    if (a.w_store) FFTAMD_LAUNCH((k_fft_plane_halves<L, INV, SPLIT, false, true>), grid, block, 0, s, a);
+    else FFTAMD_LAUNCH((k_fft_plane_halves<L, INV, SPLIT, false, false>), grid, block, 0, s, a);
   }
 }
 
@@ -225,9 +227,9 @@ static void launch_shape(const PlaneArgs& a, hipStream_t s) {
     else launch_halves<128, false, SPLIT>(a, s);
   } else {
     if (a.inverse)
-      hipLaunchKernelGGL((k_fft_plane<LY, LX, true, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+      FFTAMD_LAUNCH((k_fft_plane<LY, LX, true, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
     else
-      hipLaunchKernelGGL((k_fft_plane<LY, LX, false, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
+      FFTAMD_LAUNCH((k_fft_plane<LY, LX, false, SPLIT>), dim3((unsigned)a.nplanes), dim3(NT), 0, s, a);
   }
 }
 

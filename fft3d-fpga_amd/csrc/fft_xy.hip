@@ -25,6 +25,8 @@
 // output, so it runs out of place.
 #include <hip/hip_runtime.h>
 
+#include "launch.h"
+
 #include "fft_pass_dev.h"
 #include "fft_plane.h"
 
@@ -269,7 +271,7 @@ __global__ void __launch_bounds__(XY_NT, 4) k_fft_xy256(const PlaneArgs a) {
 
 template <bool INV, bool WL, bool WS>
 static void launch_xyw(const PlaneArgs& a, hipStream_t s, unsigned blocks) {
-  hipLaunchKernelGGL((k_fft_xy256<INV, WL, WS>), dim3(blocks), dim3(XY_NT), 0, s, a);
+  FFTAMD_LAUNCH((k_fft_xy256<INV, WL, WS>), dim3(blocks), dim3(XY_NT), 0, s, a);
 }
 
 template <bool INV>

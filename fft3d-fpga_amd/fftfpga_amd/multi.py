@@ -11,7 +11,11 @@ Schedules (plan.schedule()["mode"]): 0 one persistent launch with grid
 barriers between axes, 1 one persistent launch per axis, 2 per shape group a
 plane-kernel launch (x and y) and a line-pass launch (z); mode 2 exists when
 every transform has three axes of at least 16 points and x-y planes of at most
-128 x 128.
+128 x 128.  An untuned plan runs mode 2 where it exists, else mode 1; neither
+waits on a grid barrier.  Only check() reports a mode-0 barrier timeout.
+
+Plans and graphs made before fpga_final (or a new fpga_initialize) are stale:
+calling them raises RuntimeError; create them again.
 """
 from __future__ import annotations
 

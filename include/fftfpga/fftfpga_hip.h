@@ -11,12 +11,25 @@
  *
  * Layout conventions are those of fftfpga.h: interleaved complex, row-major,
  * last dimension contiguous, transforms of a batch contiguous.
- * d_in == d_out (in place) is allowed.
+ * d_in == d_out (in place) is allowed; input and output ranges that overlap
+ * without being the same buffer are refused (-1).
  *
- * The first call for a shape (dims, batch, flags, precision) builds its plan
- * and twiddle tables; later calls for it allocate nothing and may be captured
- * into a HIP graph (stream capture) and replayed.  Multi-transform plans make
- * their tables when created.
+ * The first call for a shape (dims, batch, flags, precision) builds its plan,
+ * its twiddle tables and the code objects of its kernels; later calls for it
+ * allocate nothing and may be captured into a HIP graph (stream capture) and
+ * replayed.  Multi-transform plans make their tables when created.
+ *
+ * Concurrency: calls are serialised on the host by one library lock, but run
+ * asynchronously on the caller's stream.  Some in-place shapes (256^3 blocks
+ * through the y-split pair, 256 x 256 planes, transposing four-step passes)
+ * run through a temporary buffer that belongs to the shape's plan: in-place
+ * calls of one shape must not overlap in time on different streams (order
+ * them on one stream, or use separate output buffers).
+ *
+ * Lifetime: fpga_final (and a new fpga_initialize) releases every plan and
+ * twiddle table.  Multi-transform plans and graphs made before then are stale:
+ * fftamd_multi_exec, fftamd_multi_tune, fftamd_graph_capture and
+ * fftamd_graph_launch refuse them with -1; destroy and create them again.
  */
 #ifndef FFTFPGA_HIP_H
 #define FFTFPGA_HIP_H
@@ -127,7 +140,10 @@ void fftamd_multi_destroy(fftamd_multi_t plan);
 void fftamd_multi_set_spin_limit(unsigned polls);
 
 /* Capture plans[0..n-1] executed in order (e.g. a forward/inverse pair) into a
- * HIP graph on a non-null stream; replay with fftamd_graph_launch. */
+ * HIP graph on a non-null stream; replay with fftamd_graph_launch.  A plan's
+ * schedule is captured as it is at capture time (an untuned plan runs mode 2
+ * where available, else mode 1; neither waits on a grid barrier).  Only
+ * fftamd_multi_check reports a mode-0 barrier timeout. */
 int fftamd_graph_capture(fftamd_graph_t *graph, fftamd_multi_t *plans, int n, void *stream);
 int fftamd_graph_launch(fftamd_graph_t graph, void *stream);
 void fftamd_graph_destroy(fftamd_graph_t graph);

@@ -69,6 +69,20 @@ def sampled(name, dims, seed, inverse, nsamp=256):
              y_at=y[idx], energy=np.vdot(y, y).real)
 
 
+def sampled_1d_bitrev(name, n, seed, inverse, nsamp=256):
+    """One n-point transform in the library's 1D output order (bit-reversed):
+    y_at[i] = X[bitrev(idx[i])] at 256 sampled output positions idx."""
+    x = datagen.uniform(n, seed)
+    X = fftn(x[None], inverse)[0]
+    lg = n.bit_length() - 1
+    rng = np.random.default_rng(seed + 2000)
+    idx = np.sort(rng.choice(n, nsamp, replace=False)).astype(np.int64)
+    idx[0] = 0
+    nat = np.array([bitrev(int(i), lg) for i in idx], np.int64)
+    np.savez(os.path.join(HERE, name), kind="sampled_1d_bitrev", n=n, seed=seed, inverse=inverse,
+             x_head=x.view(np.uint32)[:64].copy(), idx=idx, y_at=X[nat], energy=np.vdot(X, X).real)
+
+
 def main():
     nd("nd_3d_8_fwd_seed0.npz", [8, 8, 8], 1, 0, False)
     nd("nd_3d_16_inv_seed1.npz", [16, 16, 16], 1, 1, True)
@@ -80,6 +94,7 @@ def main():
     sampled("sampled_3d_64_fwd_seed0.npz", [64, 64, 64], 0, False)
     sampled("sampled_3d_256_fwd_seed1.npz", [256, 256, 256], 1, False)
     sampled("sampled_2d_4096_fwd_seed1.npz", [4096, 4096], 1, False)
+    sampled_1d_bitrev("sampled_1d_4m_fwd_seed1.npz", 1 << 22, 1, False)
 
 
 if __name__ == "__main__":

@@ -47,5 +47,30 @@ def test_harness_runs_and_verifies(args):
     r = run(*args, "-p", LIB)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Kernel Execution" in r.stdout
-    # FFTW (or MKL's FFTW3 interface) is found on the box: the SNR check must pass
-    assert "PASSED" in r.stdout or "verification skipped" in r.stdout
+    if _fftwf_api_loads():
+        # the harness finds the same library: its SNR check must run and pass
+        assert "PASSED" in r.stdout, r.stdout
+    else:
+        assert "verification skipped" in r.stdout, r.stdout
+
+
+def _fftwf_api_loads():
+    """Whether the FFTW single-precision API loads on this host from the
+    libraries the harness tries (apps/fft.cpp: libfftw3f, then MKL's FFTW3
+    interface in libmkl_rt)."""
+    import ctypes
+    for name in ("libfftw3f.so.3", "libfftw3f.so", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so"):
+        try:
+            lib = ctypes.CDLL(name)
+        except OSError:
+            continue
+        if all(hasattr(lib, f) for f in ("fftwf_plan_many_dft", "fftwf_execute", "fftwf_destroy_plan")):
+            return True
+    return False
+
+
+def test_fftwf_probe_matches_harness_search_order():
+    """The probe above tries the harness's candidates in the harness's order."""
+    src = open(os.path.join(ROOT, "fft3d-fpga_amd", "apps", "fft.cpp")).read()
+    for name in ("libfftw3f.so.3", "libfftw3f.so", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so"):
+        assert f'"{name}"' in src

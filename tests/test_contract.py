@@ -212,3 +212,23 @@ def test_python_side_size_check(fl):
         fl.fftfpgaf_c2c_3d_ddr(8, a, a, False)
     with pytest.raises(TypeError):
         fl.fftfpgaf_c2c_3d_ddr(2, np.zeros(8, np.complex128), a, False)
+
+
+def test_partially_overlapping_device_buffers_refused(fl):
+    """Device entry points refuse input and output ranges that overlap without
+    being the same buffer (the first pass may run out of place), before any
+    device work: the pointers below are never dereferenced."""
+    import ctypes
+    lib = fl.load()
+    dims = (ctypes.c_uint * 3)(64, 64, 64)
+    base = 0x10000000
+    for lo, hi in ((base, base + 8), (base + 8 * 64 ** 3 - 8, base)):
+        rc = lib.fftamd_c2c_dev(3, dims, 1, ctypes.c_void_p(lo), ctypes.c_void_p(hi), 0, 0, None)
+        assert rc == -1 and "overlap" in fl.last_error()
+        rc = lib.fftamd_c2c_weighted_dev(3, dims, 1, ctypes.c_void_p(lo), ctypes.c_void_p(hi), 0,
+                                         None, None, None)
+        assert rc == -1 and "overlap" in fl.last_error()
+    # FP64: 16-byte points, so offsets within 16 * N^3 bytes overlap
+    rc = lib.fftamd_z2z_dev(3, dims, 1, ctypes.c_void_p(base), ctypes.c_void_p(base + 8 * 64 ** 3), 0, 0,
+                            None)
+    assert rc == -1 and "overlap" in fl.last_error()

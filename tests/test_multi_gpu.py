@@ -372,3 +372,56 @@ def test_plane_schedule_split_matches_per_transform(gpu_lib, orc):
         n = int(np.prod(s))
         assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s))) <= TOL, s
     plan.close()
+
+
+def test_stale_plan_and_graph_refused_after_final(gpu_lib):
+    """fpga_final releases the twiddle tables a plan and its graph point to:
+    calling either afterwards is refused with an error instead of reading freed
+    memory; a plan made after re-initialisation works."""
+    import torch
+    from fftfpga_amd.multi import Graph, MultiPlan
+    shapes = [(32, 32, 32)] * 2 + [(64, 64, 64)]
+    host, d_in, offs = _pack(shapes, 13)
+    d_out = torch.empty_like(d_in)
+    plan = MultiPlan(shapes, d_in, d_out)
+    g = Graph([plan])
+    plan()
+    torch.cuda.synchronize()
+    gpu_lib.fpga_final()
+    try:
+        with pytest.raises(RuntimeError, match="predates"):
+            plan()
+        with pytest.raises(RuntimeError, match="predates"):
+            g()
+        with pytest.raises(RuntimeError, match="predates"):
+            plan.tune(None, reps=1)
+    finally:
+        assert gpu_lib.fpga_initialize("AMD HIP", gpu_lib.LIB_PATH, False) == 0
+    with pytest.raises(RuntimeError, match="predates"):
+        plan()  # still stale after a new initialisation
+    fresh = MultiPlan(shapes, d_in, d_out)
+    fresh()
+    fresh.check()
+    g.close()
+    plan.close()
+    fresh.close()
+
+
+def test_untuned_plan_has_no_grid_barrier(gpu_lib):
+    """A new plan runs mode 2 where it exists, else mode 1 (no grid barrier)."""
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    shapes = [(32, 32, 32)] * 2 + [(64, 64, 64)]
+    _, d_in, _ = _pack(shapes, 14)
+    d_out = torch.empty_like(d_in)
+    plan = MultiPlan(shapes, d_in, d_out)
+    assert plan.schedule()["mode"] == 2
+    plan.close()
+    shapes = [(8, 8, 8)] * 3  # planes below the plane kernel's 1024 points: no mode 2
+    _, d_in, _ = _pack(shapes, 15)
+    d_out = torch.empty_like(d_in)
+    plan = MultiPlan(shapes, d_in, d_out)
+    assert plan.schedule()["mode"] == 1
+    plan()
+    plan.check()
+    plan.close()

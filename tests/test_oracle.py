@@ -180,6 +180,25 @@ def test_package_generator_matches_oracle(orc, fl):
     assert u.real.min() >= -1 and u.real.max() < 1 and abs(u.real.mean()) < 0.01
 
 
+def test_gauss_generator_is_standard_normal(orc):
+    """orc_fill_gauss pinned by what N(0,1) fixes, independently of its Python
+    twin: over 10^6 draws per component the mean and variance lie within 5
+    standard errors of 0 and 1, the fourth moment of 3, re and im are
+    uncorrelated, and a Kolmogorov-Smirnov test against the normal CDF does not
+    reject (a wrong Box-Muller scale, a dropped sqrt(-2 ln u) or a cosine used
+    for both components fails these)."""
+    from scipy import stats
+    n = 1_000_000
+    g = orc.gauss(n, 17)
+    for comp in (g.real.astype(np.float64), g.imag.astype(np.float64)):
+        se = 1.0 / np.sqrt(n)
+        assert abs(comp.mean()) < 5 * se
+        assert abs(comp.var() - 1.0) < 5 * np.sqrt(2.0) * se
+        assert abs(np.mean(comp ** 4) - 3.0) < 5 * np.sqrt(96.0) * se
+        assert stats.kstest(comp[:200_000], "norm").pvalue > 1e-4
+    assert abs(np.corrcoef(g.real, g.imag)[0, 1]) < 5.0 / np.sqrt(n)
+
+
 def test_error_metrics(orc):
     ref = orc.gauss(4096, 2).astype(np.complex128)
     got = (ref * (1 + 1e-6)).astype(np.complex64)
@@ -212,6 +231,13 @@ def test_oracle_matches_golden(orc, name):
         x = orc.uniform(int(np.prod(dims)), int(z["seed"]))
         assert np.array_equal(x.view(np.uint32)[: z["x_head"].size], z["x_head"])
         got = orc.fftnd(x, dims, inverse=inv).reshape(-1)
+        assert rel(got[z["idx"]], z["y_at"]) < 1e-13
+        assert abs(np.vdot(got, got).real - float(z["energy"])) < 1e-10 * float(z["energy"])
+    elif kind == "sampled_1d_bitrev":
+        n = int(z["n"])
+        x = orc.uniform(n, int(z["seed"]))
+        assert np.array_equal(x.view(np.uint32)[: z["x_head"].size], z["x_head"])
+        got = orc.c2c_1d_expected(x, n, 1, inverse=inv).reshape(-1)
         assert rel(got[z["idx"]], z["y_at"]) < 1e-13
         assert abs(np.vdot(got, got).real - float(z["energy"])) < 1e-10 * float(z["energy"])
     else:

@@ -126,24 +126,25 @@ def run_plan(passes, final_copy, x, inverse, aliased=False):
             bufs[p["dst"]][:] = X.reshape(-1)
             cur = p["dst"]
             continue
-        if p["mode"] in (4, 5):  # n^3 blocks, y split (n/4) x 4 (y = a + 4 b, ky = k + (n/4) d)
+        if p["mode"] in (4, 5):  # n^3 blocks, y split q x A (y = a + A b, ky = k + q d)
             def dft(v, ax):
                 return np.fft.ifft(v, axis=ax) * v.shape[ax] if inverse else np.fft.fft(v, axis=ax)
             assert p["src"] == cur
             n = 1 << p["lg"]
-            q = n // 4
-            assert n in (128, 256)
+            assert n in (128, 256, 512)
+            A = 8 if n == 512 else 4
+            q = n // A
             if p["mode"] == 4:  # x, then over b; the column DFT of residue a to row q a + k
                 assert p["transposing"] and p["src"] != p["dst"]
                 assert p["in_o"] == n * n and p["nouter"] * n * n == total
                 pl = bufs[p["src"]].reshape(p["nouter"], n, n)
                 res = np.empty_like(pl)
-                for a in range(4):
-                    res[:, q * a:q * a + q, :] = dft(dft(pl[:, a::4, :], 2), 1)
+                for a in range(A):
+                    res[:, q * a:q * a + q, :] = dft(dft(pl[:, a::A, :], 2), 1)
             else:  # row k + q a times Wn^{a k}, over a to rows k + q d, then z
                 assert p["in_o"] == n ** 3 and p["nouter"] * n ** 3 == total
-                blocks = bufs[p["src"]].reshape(p["nouter"], n, 4, q, n)
-                ak = np.arange(4)[:, None] * np.arange(q)[None, :]
+                blocks = bufs[p["src"]].reshape(p["nouter"], n, A, q, n)
+                ak = np.arange(A)[:, None] * np.arange(q)[None, :]
                 w = np.exp((2j if inverse else -2j) * np.pi * ak / n)
                 res = dft(dft(blocks * w[None, None, :, :, None], 2), 1)
             bufs[p["dst"]][:] = res.reshape(-1)
@@ -694,3 +695,138 @@ def test_cube_split_index_maps(n):
     got = emulate_cube_b(emulate_cube_a(x, n), n)
     ref = np.fft.fftn(x)
     assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+# ---- 512^3 blocks: the y-split pair with y = a + 8 b (fft_cube512.hip) --------------------
+
+def test_c512_split_formula():
+    """The factorisation both kernels implement, on 512-point lines:
+    X[k + 64 d] = sum_a W8^{a d} W512^{a k} T_a[k], T_a[k] = sum_b x[a + 8 b] W64^{b k}."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(512) + 1j * rng.standard_normal(512)
+    T = np.fft.fft(x.reshape(64, 8).T, axis=1)  # [a, k]
+    a, k = np.arange(8)[:, None], np.arange(64)[None, :]
+    X = np.fft.fft(T * np.exp(-2j * np.pi * a * k / 512), axis=0)  # [d, k]
+    assert np.allclose(X.reshape(-1), np.fft.fft(x), atol=1e-10)  # index k + 64 d
+
+
+def emulate_c512_a(plane, res):
+    """Pass A on one 512 x 512 plane for residue res: thread (c, tau) loads points
+    tau + 16 r of row y = res + 8 c and does the radix-32 stage; round R writes
+    its sub-transforms 16 R .. 16 R + 15 to LDS [(c 16 + k) 17 + tau] and thread
+    (c, tau') runs radix 16 over tau for k1 = 16 R + tau' (ending with kx = k1 +
+    32 m); column half R puts point (row c, column tau + 16 m) at LDS
+    [c 257 + tau + 16 m] and column thread (cc, tq) stores point k = tq + 4 r of
+    column kx = 16 R + (cc & 15) + 32 (cc >> 4) to row 64 res + k.  Returns the
+    64 output rows (NaN where not written)."""
+    tid = np.arange(1024)
+    c, tau = tid >> 4, tid & 15
+    r = np.arange(32)
+    v = plane[(res + 8 * c)[:, None], tau[:, None] + 16 * r[None, :]]  # [tid, r]
+    Y = np.fft.fft(v, axis=1)  # Y_tau[k1]
+    o = np.empty((2, 1024, 16), complex)
+    for R in range(2):
+        lds = np.full(64 * 16 * 17, np.nan + 0j)
+        for k in range(16):
+            idx = (c * 16 + k) * 17 + tau
+            assert np.isnan(lds[idx]).all(), "x-round LDS slot written twice"
+            lds[idx] = Y[:, 16 * R + k]
+        y = lds[(c * 16 + tau)[:, None] * 17 + np.arange(16)[None, :]]  # [tid, t]
+        assert not np.isnan(y).any(), "x-round LDS slot read before written"
+        k1 = 16 * R + tau
+        y = y * np.exp(-2j * np.pi * (k1[:, None] * np.arange(16)[None, :]) / 512)
+        o[R] = np.fft.fft(y, axis=1)  # X[k1 + 32 m]
+    out = np.full((64, 512), np.nan + 0j)
+    cc, tq = tid & 255, tid >> 8
+    for R in range(2):
+        lds = np.full(64 * 257, np.nan + 0j)
+        for m in range(16):
+            idx = c * 257 + tau + 16 * m
+            assert np.isnan(lds[idx]).all(), "column LDS slot written twice"
+            lds[idx] = o[R][:, m]
+        cols = lds[np.arange(64)[None, :] * 257 + np.arange(256)[:, None]]  # [col, b]
+        assert not np.isnan(cols).any(), "column LDS slot left empty"
+        colfft = np.fft.fft(cols, axis=1)  # [col, k]
+        kk = tq[:, None] + 4 * np.arange(16)[None, :]
+        kx = 16 * R + (cc & 15) + 32 * (cc >> 4)
+        assert np.isnan(out[kk, kx[:, None]]).all(), "output written twice"
+        out[kk, kx[:, None]] = colfft[cc[:, None], kk]
+    return out
+
+
+def test_c512_pass_a_maps():
+    rng = np.random.default_rng(21)
+    for _ in range(2):
+        plane = rng.standard_normal((512, 512)) + 1j * rng.standard_normal((512, 512))
+        full = np.full((512, 512), np.nan + 0j)
+        for res in range(8):
+            full[64 * res:64 * res + 64] = emulate_c512_a(plane, res)
+        assert not np.isnan(full).any()
+        # row 64 a + k holds T_a[k] = sum_b FFT_x(x[a + 8 b]) W64^{b k}
+        ref = np.fft.fft(np.fft.fft(plane, axis=1).reshape(64, 8, 512), axis=0)  # [k, a, kx]
+        ref = ref.transpose(1, 0, 2).reshape(512, 512)
+        assert np.linalg.norm(full - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def emulate_c512_b(tin, k):
+    """Pass B for one tile: tin[z, a, x] is row k + 64 a, column x of the tile's
+    8 columns in plane z.  Thread (x, zs) loads planes zs + 128 j of its column,
+    applies W512^{a k}, the 8-point DFT over a, the 4-point DFT over j and
+    W512^{zs kq}; half hh writes line 32 dd + 8 kq + x (d = 4 hh + dd), slot zs
+    to LDS [zs 129 + line]; line thread (cz, tz) ends with point m = tz + 8 r,
+    stored at plane kq + 4 m of row k + 64 d.  Returns out[kz, d, x]."""
+    tid = np.arange(1024)
+    x, zs = tid & 7, tid >> 3
+    j = np.arange(4)
+    t = tin[(zs[:, None] + 128 * j[None, :]), :, x[:, None]]  # [tid, j, a]
+    t = t * np.exp(-2j * np.pi * np.arange(8) * k / 512)[None, None, :]
+    t = np.fft.fft(t, axis=2)  # [tid, j, d]
+    t = np.fft.fft(t, axis=1)  # [tid, kq, d]
+    t = t * np.exp(-2j * np.pi * (zs[:, None] * np.arange(4)[None, :]) / 512)[:, :, None]
+    out = np.full((512, 8, 8), np.nan + 0j)
+    cz, tz = tid & 127, tid >> 7
+    for hh in range(2):
+        lds = np.full(128 * 129, np.nan + 0j)
+        for dd in range(4):
+            for kq in range(4):
+                idx = zs * 129 + 32 * dd + 8 * kq + x
+                assert np.isnan(lds[idx]).all(), "z LDS slot written twice"
+                lds[idx] = t[:, kq, 4 * hh + dd]
+        lines = lds[np.arange(128)[None, :] * 129 + np.arange(128)[:, None]]  # [line, zs]
+        assert not np.isnan(lines).any(), "z LDS slot left empty"
+        zf = np.fft.fft(lines, axis=1)  # [line, m]
+        xx, kq, dd = cz & 7, (cz >> 3) & 3, cz >> 5
+        m = tz[:, None] + 8 * np.arange(16)[None, :]
+        kz = kq[:, None] + 4 * m
+        sel = out[kz, (4 * hh + dd)[:, None], xx[:, None]]
+        assert np.isnan(sel).all(), "output written twice"
+        out[kz, (4 * hh + dd)[:, None], xx[:, None]] = zf[cz[:, None], m]
+    return out
+
+
+@pytest.mark.parametrize("k", [0, 37])
+def test_c512_pass_b_maps(k):
+    rng = np.random.default_rng(31 + k)
+    tin = rng.standard_normal((512, 8, 8)) + 1j * rng.standard_normal((512, 8, 8))
+    got = emulate_c512_b(tin, k)
+    assert not np.isnan(got).any()
+    # X[kz][k + 64 d] = sum_z W512^{z kz} sum_a W8^{a d} W512^{a k} T[z][k + 64 a]
+    ref = np.fft.fft(np.fft.fft(tin * np.exp(-2j * np.pi * np.arange(8) * k / 512)[None, :, None], axis=1),
+                     axis=0)
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_c512_pass_b_block_mapping():
+    """blockIdx -> (k, column group): every tile of a block once, and the two
+    8-column groups of a 128-byte segment on one XCD (blocks b and b + 8)."""
+    b = np.arange(2 * 4096)
+    P = (b >> 4) * 8 + (b & 7)
+    xg = 2 * (P & 31) + ((b >> 3) & 1)
+    k = (P >> 5) & 63
+    blk = P >> 11
+    key = (blk * 64 + k) * 64 + xg
+    assert np.array_equal(np.sort(key), np.arange(2 * 4096))
+    seg = (blk * 64 + k) * 32 + xg // 2
+    for s in np.unique(seg)[:200]:
+        members = b[seg == s]
+        assert len(members) == 2 and members[1] - members[0] == 8 and members[0] % 8 == members[1] % 8
