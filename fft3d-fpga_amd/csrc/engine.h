@@ -44,6 +44,14 @@ int host_c2c_pipelined(int rank, const size_t* dims, size_t count, const void* h
 int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
             unsigned flags, int prec, void* stream);
 
+// Bytes of the static device array behind dev_c2c_noalloc: every FP32 twiddle
+// table (about 150 KiB) and the temporary buffer of in-place shapes.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+// FP32 dev_c2c that allocates no device memory on any call: twiddle tables and
+// the temporary buffer come from that static array (see fftfpga_hip.h).
+int dev_c2c_noalloc(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
+                    unsigned flags, void* stream);
+
 // FP32 device transform with real weights (one per point of a transform,
 // broadcast over the batch) fused into the first pass's load (w_load) and the
 // last pass's store (w_store); either may be null.

@@ -86,9 +86,14 @@ struct Plan {
   bool final_copy = false;
   bool needs_tmp = false;
   void* tmp = nullptr;
+  // allocation-free plan: twiddle tables and the temporary buffer live in the
+  // device's static scratch array (never freed, never allocated)
+  bool noalloc = false;
 };
 
-using PlanKey = std::tuple<int, std::vector<size_t>, size_t, unsigned, int, bool>;
+// (rank, dims padded with 0, batch, flags, precision, aliased, allocation-free):
+// trivially copyable, so a lookup allocates nothing
+using PlanKey = std::tuple<int, size_t, size_t, size_t, size_t, unsigned, int, bool, bool>;
 
 struct State {
   std::recursive_mutex mu;
@@ -266,6 +271,79 @@ static int get_direct(long long M, const void** out) {
     it = st.tw_direct.emplace(M, d).first;
   }
   *out = it->second;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free entry point (fftamd_c2c_dev_noalloc): a static device array
+// holds every FP32 twiddle table a plan can read and the temporary buffer of
+// in-place shapes, so a call allocates no device memory.  In host code the
+// array's name is only a handle for hipGetSymbolAddress: the address kept per
+// device (na_dev[dev].base) is what kernels and copies receive.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+struct NoallocDev {
+  unsigned char* base = nullptr;  // device address of `scratch` on this device
+  size_t stock[14] = {};          // byte offset of the Stockham table of 2^lg points
+  size_t fs_lo[23] = {}, fs_hi[23] = {};  // four-step tables of modulus 2^lg
+  size_t w256 = 0, w512 = 0;      // W256^t, t < 256; W512^t, t < 512
+  size_t tmp_off = 0;             // the temporary buffer: [tmp_off, NOALLOC_SCRATCH_BYTES)
+  bool ready = false;
+};
+constexpr int NOALLOC_MAX_DEV = 64;
+static NoallocDev na_dev[NOALLOC_MAX_DEV];
+
+// Once per device, with that device current: resolve the array's address and
+// write every table into it (the same long-double tables as the allocating
+// path, so both paths give the same bits).
+static int noalloc_setup(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) {
+    set_error("device %d out of range for the allocation-free scratch", dev);
+    return -1;
+  }
+  NoallocDev& d = na_dev[dev];
+  if (d.ready) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) {
+    set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    return -1;
+  }
+  std::vector<float> img;
+  auto put = [&](const std::vector<float>& v) {
+    img.resize((img.size() + 63) / 64 * 64, 0.f);  // 256-byte aligned tables
+    const size_t off = img.size() * sizeof(float);
+    img.insert(img.end(), v.begin(), v.end());
+    return off;
+  };
+  for (int lg = 1; lg <= 13; ++lg)
+    if (sched_np(1 << lg) > 1) d.stock[lg] = put(stockham_table<float>(1 << lg));
+  for (int lg = 2; lg <= 22; ++lg) {
+    std::vector<float> lo, hi;
+    fourstep_tables<float>(1LL << lg, lo, hi);
+    d.fs_lo[lg] = put(lo);
+    d.fs_hi[lg] = put(hi);
+  }
+  for (int M : {256, 512}) {
+    std::vector<float> w(2 * (size_t)M);
+    for (int t = 0; t < M; ++t) {
+      long double c, sn;
+      twiddle_ld(t, M, &c, &sn);
+      w[2 * t] = (float)c;
+      w[2 * t + 1] = (float)sn;
+    }
+    (M == 256 ? d.w256 : d.w512) = put(w);
+  }
+  const size_t bytes = img.size() * sizeof(float);
+  d.tmp_off = (bytes + 255) / 256 * 256;
+  if (d.tmp_off >= NOALLOC_SCRATCH_BYTES) {
+    set_error("allocation-free scratch too small for the twiddle tables");
+    return -1;
+  }
+  HIP_TRY(hipMemcpy(p, img.data(), bytes, hipMemcpyHostToDevice));
+  d.base = static_cast<unsigned char*>(p);
+  d.ready = true;
   return 0;
 }
 
@@ -566,6 +644,42 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   return 0;
 }
 
+// Twiddle tables of a plan: the cached allocations, or the static scratch of
+// an allocation-free plan.
+static int tab_stockham(const Plan& plan, int L, const void** out) {
+  if (!plan.noalloc) return get_stockham(plan.prec, L, out);
+  const NoallocDev& d = na_dev[S().dev];
+  const int lg = ilog2c(L);
+  if (plan.prec != PREC_F32 || lg < 1 || lg > 13 || !d.ready) {
+    set_error("no allocation-free Stockham table for %d points", L);
+    return -1;
+  }
+  *out = d.base + d.stock[lg];
+  return 0;
+}
+static int tab_fourstep(const Plan& plan, long long M, const void** lo, const void** hi) {
+  if (!plan.noalloc) return get_fourstep(plan.prec, M, lo, hi);
+  const NoallocDev& d = na_dev[S().dev];
+  const int lg = ilog2c(M);
+  if (plan.prec != PREC_F32 || lg < 2 || lg > 22 || !d.ready) {
+    set_error("no allocation-free four-step table for modulus %lld", M);
+    return -1;
+  }
+  *lo = d.base + d.fs_lo[lg];
+  *hi = d.base + d.fs_hi[lg];
+  return 0;
+}
+static int tab_direct(const Plan& plan, long long M, const void** out) {
+  if (!plan.noalloc) return get_direct(M, out);
+  const NoallocDev& d = na_dev[S().dev];
+  if (!d.ready || (M != 256 && M != 512)) {
+    set_error("no allocation-free table of W_%lld", M);
+    return -1;
+  }
+  *out = d.base + (M == 256 ? d.w256 : d.w512);
+  return 0;
+}
+
 static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipStream_t stream,
                     const float* w_load = nullptr, const float* w_store = nullptr);
 
@@ -584,21 +698,34 @@ static int plan_prepare(Plan& plan) {
 }
 
 static int get_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
-                    Plan** out) {
+                    Plan** out, bool noalloc = false) {
   State& st = S();
-  PlanKey key(rank, std::vector<size_t>(dims, dims + rank), batch, flags & (FLAG_BITREV_1D | FLAG_SPLIT_MFMA | FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT),
-              prec, aliased);
+  PlanKey key(rank, dims[0], rank > 1 ? dims[1] : 0, rank > 2 ? dims[2] : 0, batch,
+              flags & (FLAG_BITREV_1D | FLAG_SPLIT_MFMA | FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT), prec,
+              aliased, noalloc);
   auto it = st.plans.find(key);
   if (it == st.plans.end()) {
     Plan p;
     if (build_plan(rank, dims, batch, flags, prec, aliased, p)) return -1;
+    p.noalloc = noalloc;
     if (p.needs_tmp) {
       const size_t bytes = p.total * (prec == PREC_F32 ? 8 : 16);
-      HIP_TRY(hipMalloc(&p.tmp, bytes));
+      if (noalloc) {
+        const NoallocDev& d = na_dev[st.dev];
+        if (!d.ready || bytes > NOALLOC_SCRATCH_BYTES - d.tmp_off) {
+          set_error("this in-place shape needs a %zu-byte temporary buffer; the allocation-free scratch "
+                    "holds %zu bytes (call out of place, or use fftamd_c2c_dev)",
+                    bytes, (size_t)(NOALLOC_SCRATCH_BYTES - d.tmp_off));
+          return -1;
+        }
+        p.tmp = d.base + d.tmp_off;
+      } else {
+        HIP_TRY(hipMalloc(&p.tmp, bytes));
+      }
     }
     it = st.plans.emplace(key, std::move(p)).first;
     if (plan_prepare(it->second)) {
-      if (it->second.tmp) (void)hipFree(it->second.tmp);
+      if (it->second.tmp && !it->second.noalloc) (void)hipFree(it->second.tmp);
       st.plans.erase(it);
       return -1;
     }
@@ -628,8 +755,8 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       xa.out = ptr(p.dst);
       xa.nplanes = p.nouter;
       xa.plane_stride = p.in_o;
-      if (sched_np(LX) > 1 && get_stockham(plan.prec, LX, &xa.tw_x)) return -1;
-      if (sched_np(LY) > 1 && get_stockham(plan.prec, LY, &xa.tw_y)) return -1;
+      if (sched_np(LX) > 1 && tab_stockham(plan, LX, &xa.tw_x)) return -1;
+      if (sched_np(LY) > 1 && tab_stockham(plan, LY, &xa.tw_y)) return -1;
       xa.w_load = pi == 0 ? w_load : nullptr;
       xa.w_store = pi + 1 == plan.passes.size() ? w_store : nullptr;
       xa.w_mask = (unsigned long long)plan.per - 1ULL;
@@ -661,9 +788,9 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       ca.n = 1 << p.lg;
       ca.nplanes = A ? p.nouter : p.nouter * (unsigned long long)ca.n;
       // 512: pass A's columns are 64-point lines, pass B's z lines 4 x 128
-      if (get_stockham(PREC_F32, ca.n == 512 ? (A ? 64 : 128) : ca.n, &ca.tw)) return -1;
-      if (get_direct(256, &ca.w256)) return -1;
-      if (ca.n == 512 && get_direct(512, &ca.w512)) return -1;
+      if (tab_stockham(plan, ca.n == 512 ? (A ? 64 : 128) : ca.n, &ca.tw)) return -1;
+      if (tab_direct(plan, 256, &ca.w256)) return -1;
+      if (ca.n == 512 && tab_direct(plan, 512, &ca.w512)) return -1;
       // pass A is always the first pass of its plan; pass B takes the store
       // weights when it is the last
       ca.w_load = A && pi == 0 ? w_load : nullptr;
@@ -696,9 +823,9 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       ca.in_o = p.in_o;
       ca.ngroups = (unsigned)(p.ncol / 16);
       ca.nitems = p.nouter * ca.ngroups;
-      if (get_stockham(PREC_F32, 256, &ca.tw)) return -1;
-      if (get_direct(256, &ca.w256)) return -1;
-      if (get_fourstep(PREC_F32, 4096, &ca.ptw_lo, &ca.ptw_hi)) return -1;
+      if (tab_stockham(plan, 256, &ca.tw)) return -1;
+      if (tab_direct(plan, 256, &ca.w256)) return -1;
+      if (tab_fourstep(plan, 4096, &ca.ptw_lo, &ca.ptw_hi)) return -1;
       ca.ptw_hb = fourstep_hb(4096);
       ca.w_load = pi == 0 ? w_load : nullptr;
       ca.w_store = pi + 1 == plan.passes.size() ? w_store : nullptr;
@@ -744,10 +871,10 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
     a.kbits = p.kbits;
     a.inverse = inverse ? 1 : 0;
     if (sched_np(L) > 1) {
-      if (get_stockham(plan.prec, L, &a.tw)) return -1;
+      if (tab_stockham(plan, L, &a.tw)) return -1;
     }
     if (p.epi == EPI_TWIDDLE) {
-      if (get_fourstep(plan.prec, p.M, &a.ptw_lo, &a.ptw_hi)) return -1;
+      if (tab_fourstep(plan, p.M, &a.ptw_lo, &a.ptw_hi)) return -1;
       a.ptw_hb = fourstep_hb(p.M);
       a.ptw_mmask = (unsigned)(p.M - 1);
       a.ptw_cshift = p.cshift;
@@ -1411,7 +1538,7 @@ static void release_locked() {
   pipe_release(st.pipe);
   st.pipe = nullptr;
   for (auto& kv : st.plans)
-    if (kv.second.tmp) (void)hipFree(kv.second.tmp);
+    if (kv.second.tmp && !kv.second.noalloc) (void)hipFree(kv.second.tmp);
   st.plans.clear();
   for (auto& kv : st.tw_stockham) (void)hipFree(kv.second);
   st.tw_stockham.clear();
@@ -1445,6 +1572,7 @@ int runtime_init(int dev, bool svm) {
   HIP_TRY(hipSetDevice(dev));
   HIP_TRY(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
   for (auto& e : st.ev) HIP_TRY(hipEventCreate(&e));
+  if (noalloc_setup(dev)) return -1;  // the allocation-free entry point's scratch
   st.dev = dev;
   st.svm = svm;
   st.ready = true;
@@ -1482,7 +1610,7 @@ void plan_cache_clear() {
   std::lock_guard<std::recursive_mutex> lk(st.mu);
   if (st.ready) (void)hipDeviceSynchronize();
   for (auto& kv : st.plans)
-    if (kv.second.tmp) (void)hipFree(kv.second.tmp);
+    if (kv.second.tmp && !kv.second.noalloc) (void)hipFree(kv.second.tmp);
   st.plans.clear();
 }
 
@@ -1847,6 +1975,18 @@ int dev_c2c_weighted(int rank, const size_t* dims, size_t batch, const void* din
   if (get_plan(rank, dims, batch, 0u, PREC_F32, din == dout, &plan)) return -1;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);  // NULL = the HIP null stream
   return run_plan(*plan, din, dout, inverse, s, w_load, w_store);
+}
+
+int dev_c2c_noalloc(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,
+                    unsigned flags, void* stream) {
+  if (!check_dims(rank, dims, batch) || check_dev_buffers(rank, dims, batch, din, dout, 8)) return -1;
+  State& st = S();
+  std::lock_guard<std::recursive_mutex> lk(st.mu);
+  if (runtime_ensure_device()) return -1;
+  if (noalloc_setup(st.dev)) return -1;
+  Plan* plan = nullptr;
+  if (get_plan(rank, dims, batch, flags, PREC_F32, din == dout, &plan, true)) return -1;
+  return run_plan(*plan, din, dout, inverse, reinterpret_cast<hipStream_t>(stream));
 }
 
 int dev_c2c(int rank, const size_t* dims, size_t batch, const void* din, void* dout, bool inverse,

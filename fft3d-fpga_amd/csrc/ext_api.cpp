@@ -33,6 +33,13 @@ int fftamd_c2c_dev(int rank, const unsigned* dims, unsigned batch, const float2*
   return fftamd::dev_c2c(rank, d, batch, d_in, d_out, inverse != 0, flags, 0, stream);
 }
 
+int fftamd_c2c_dev_noalloc(int rank, const unsigned* dims, unsigned batch, const float2* d_in, float2* d_out,
+                           int inverse, unsigned flags, void* stream) {
+  size_t d[3];
+  if (!dims_ok(rank, dims, batch, d)) return -1;
+  return fftamd::dev_c2c_noalloc(rank, d, batch, d_in, d_out, inverse != 0, flags, stream);
+}
+
 int fftamd_z2z_dev(int rank, const unsigned* dims, unsigned batch, const double2* d_in,
                    double2* d_out, int inverse, unsigned flags, void* stream) {
   size_t d[3];

@@ -68,6 +68,7 @@ SIGNATURES = {
     "fftfpgaf_c2c_3d_ddr_svm_batch": (FpgaT, [_u, _vp, _vp, _b, _u]),
     # fftfpga_hip.h
     "fftamd_c2c_dev": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _u, _vp]),
+    "fftamd_c2c_dev_noalloc": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _u, _vp]),
     "fftamd_z2z_dev": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _u, _vp]),
     "fftamd_c2c_weighted_dev": (_i, [_i, ctypes.POINTER(_u), _u, _vp, _vp, _i, _vp, _vp, _vp]),
     "fftfpgaf_c2c_3d_ddr_dev": (_i, [_u, _vp, _vp, _b, _u, _vp]),

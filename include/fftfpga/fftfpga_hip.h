@@ -67,6 +67,22 @@ extern "C" {
 int fftamd_c2c_dev(int rank, const unsigned *dims, unsigned batch, const float2 *d_in, float2 *d_out,
                    int inverse, unsigned flags, void *stream);
 
+/* fftamd_c2c_dev that allocates no device memory on any call, its first
+ * included.  No counterpart in the reference: it is for callers that cannot
+ * allocate -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Every FP32 twiddle table and the temporary buffer of
+ * in-place shapes live in one static device array of 64 MiB per GPU (held
+ * until the process ends), set up when the library initialises on a device
+ * (fpga_initialize, or the first device call).  Shapes whose in-place
+ * temporary buffer exceeds the array (about 63.8 MiB: e.g. 256^3 in place)
+ * are refused with -1; call them out of place or through fftamd_c2c_dev.
+ * Every call on a GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap in time on one GPU (different streams) race.
+ * The host keeps one small plan record per new shape; a repeated shape
+ * allocates nothing at all.  Results equal fftamd_c2c_dev's. */
+int fftamd_c2c_dev_noalloc(int rank, const unsigned *dims, unsigned batch, const float2 *d_in,
+                           float2 *d_out, int inverse, unsigned flags, void *stream);
+
 /* Double precision twin of fftamd_c2c_dev. */
 int fftamd_z2z_dev(int rank, const unsigned *dims, unsigned batch, const double2 *d_in,
                    double2 *d_out, int inverse, unsigned flags, void *stream);
