@@ -407,7 +407,7 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   int top = rank - 1;
   const long long CN = rank >= 3 ? (long long)dims[rank - 1] : 0;
   if (rank >= 3 && prec == PREC_F32 && !plan.split && !(flags & (FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT)) &&
-      (CN == 256 || CN == 128) && (long long)dims[rank - 2] == CN && (long long)dims[rank - 3] == CN) {
+      (CN == 512 || CN == 256 || CN == 128) && (long long)dims[rank - 2] == CN && (long long)dims[rank - 3] == CN) {
     unsigned long long O = batch;
     for (int d = 0; d < rank - 3; ++d) O *= dims[d];
     PassDesc A;

@@ -585,3 +585,93 @@ def test_c2c_dev_pair_replays_from_cuda_graph(gpu_lib, orc, dims, batch):
         assert torch.equal(y, eager_y), f"replay {r}: inverse differs from the eager result"
     ref = x.cpu().numpy().reshape(-1)
     assert orc.rel_l2(y.cpu().numpy().reshape(-1) / (n // batch), ref) <= TOL
+
+
+# ---- 512^3 blocks: the y-split pair with y = a + 8 b ----------------------------------------
+
+def test_cube512_vs_oracle(gpu_lib, orc):
+    """The 512^3 pair (x and the 64-point y stage over rows y = a + 8 b; then
+    the 8-point y stage and z) against the FP64 oracle, both directions, out of
+    place and in place (pass A through the temporary buffer), and against the
+    three line passes."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    N = 512
+    x = orc.uniform(N ** 3, 5120)
+    xt = torch.from_numpy(x.reshape(N, N, N)).cuda()
+    for inv in (False, True):
+        ref = orc.fftnd(x, [N, N, N], inverse=inv)
+        y = c2c_dev(xt, 3, inverse=inv)
+        torch.cuda.synchronize()
+        err = orc.rel_l2(y.cpu().numpy(), ref)
+        del y
+        yl = c2c_dev(xt, 3, inverse=inv, cube_split=False)
+        torch.cuda.synchronize()
+        errl = orc.rel_l2(yl.cpu().numpy(), ref)
+        del yl
+        z = xt.clone()
+        c2c_dev(z, 3, inverse=inv, out=z)
+        torch.cuda.synchronize()
+        errz = orc.rel_l2(z.cpu().numpy(), ref)
+        del z
+        print(f"512^3 inv={inv} pair={err:.3e} line passes={errl:.3e} in place={errz:.3e}")
+        assert err <= TOL and errl <= TOL and errz <= TOL
+    assert np.array_equal(xt.cpu().numpy().reshape(-1), x)  # input untouched
+
+
+def test_cube512_single_tones_exact(gpu_lib):
+    """Single complex exponentials land exactly on their bins through the 512^3
+    pair: every y residue class (a = ky mod 8) and output row block (d = ky // 64),
+    both directions, generated on the device."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    N = 512
+    i = torch.arange(N, dtype=torch.float64, device="cuda")
+    tones = [(0, 0, 0), (1, 2, 3), (N - 1, 64 + 7, N // 2), (17, 8 * 63 + 5, 5), (N // 2, N - 1, N - 2),
+             (300, 130, 511)]
+    for (kz, ky, kx) in tones:
+        x = (torch.exp(2j * np.pi * kz * i / N).reshape(N, 1, 1) *
+             torch.exp(2j * np.pi * ky * i / N).reshape(1, N, 1) *
+             torch.exp(2j * np.pi * kx * i / N).reshape(1, 1, N)).to(torch.complex64)
+        for inv in (False, True):
+            y = c2c_dev(x, 3, inverse=inv)
+            k = ((-kz) % N, (-ky) % N, (-kx) % N) if inv else (kz, ky, kx)
+            idx = int(np.ravel_multi_index(k, (N, N, N)))
+            flat = y.reshape(-1)
+            assert int(torch.argmax(flat.abs())) == idx, (kz, ky, kx, inv)
+            peak = complex(flat[idx].item())
+            flat[idx] = 0
+            rest = flat.abs().max().item()
+            print(f"512^3 tone {(kz, ky, kx)} inv={inv} peak={peak} max_rest={rest:.3e}")
+            assert abs(peak - N ** 3) <= 1e-4 * N ** 3
+            assert rest <= 1e-3 * N ** 1.5
+            del y, flat
+        del x
+
+
+def test_cube512_weights(gpu_lib, orc):
+    """Weights fused into the 512^3 pair: out = w_out . F(w_in . x) (pass A's
+    loads, pass B's stores) against the oracle."""
+    import ctypes
+    import torch
+    lib = gpu_lib.load()
+    N = 512
+    n = N ** 3
+    x = orc.uniform(n, 5121)
+    w_in = np.abs(orc.uniform(n, 5122).real).astype(np.float32)
+    w_out = np.abs(orc.uniform(n, 5123).real).astype(np.float32)
+    xt = torch.from_numpy(x).cuda()
+    wi, wo = torch.from_numpy(w_in).cuda(), torch.from_numpy(w_out).cuda()
+    y = torch.empty_like(xt)
+    dims = (ctypes.c_uint * 3)(N, N, N)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for inv in (False, True):
+        rc = lib.fftamd_c2c_weighted_dev(3, dims, 1, ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                         1 if inv else 0, ctypes.c_void_p(wi.data_ptr()),
+                                         ctypes.c_void_p(wo.data_ptr()), s)
+        assert rc == 0, gpu_lib.last_error()
+        torch.cuda.synchronize()
+        ref = w_out * orc.fftnd((w_in * x).astype(np.complex64), [N, N, N], inverse=inv).reshape(-1)
+        err = orc.rel_l2(y.cpu().numpy(), ref)
+        print(f"512^3 weighted inv={inv} rel_l2={err:.3e}")
+        assert err <= TOL

@@ -261,6 +261,13 @@ def test_plan_shapes_of_headline_configs(fl):
     # double precision and split mode keep the separate passes
     assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, prec=1)[0]] == [0, 1, 1]
     assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, flags=2)[0]] == [0, 1, 1]
+    # 512^3 blocks: the y-split pair with y = a + 8 b, 1024-thread workgroups
+    p, fc = describe(fl, [512, 512, 512], 1)
+    assert [(q["mode"], q["lg"], q["NT"], q["C"]) for q in p] == [(4, 9, 1024, 64), (5, 9, 1024, 8)] and not fc
+    assert [(q["src"], q["dst"]) for q in p] == [(0, 1), (1, 1)]
+    p, fc = describe(fl, [512, 512, 512], 1, aliased=True)
+    assert [(q["src"], q["dst"]) for q in p] == [(1, 2), (2, 1)] and not fc
+    assert [q["mode"] for q in describe(fl, [512, 512, 512], 1, flags=8)[0]] == [0, 1, 1]
     # 1D 2^22 bit-reversed: four-step, strided 512 then contiguous 8192
     p, fc = describe(fl, [1 << 22], 1, 1)
     assert [(q["lg"], q["mode"], q["epi"]) for q in p] == [(9, 1, 1), (13, 0, 2)]
