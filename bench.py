@@ -219,8 +219,9 @@ def cpu_fftwf(dims, seed, reps=5):
 def split_512(steps):
     """Split-precision mode at 512^3 on U[-1,1) plus +-1e-3 spikes on 0.1% of
     the entries (splitmix64 seed 3): forward transform time of the FP32 path
-    and of the split path (radix-16 stages on FP16 matrix cores), and the
-    split result's relative L2 distance from the FP32 path.  (Accuracy against
+    (the y-split pair), of FP32 through the three line passes (the passes the
+    split mode runs), and of the split path (radix-16 stages on FP16 matrix
+    cores), and the split result's relative L2 distance from the FP32 path.  (Accuracy against
     the FP64 oracle is checked in the GPU tests.)"""
     import torch
 
@@ -231,17 +232,18 @@ def split_512(steps):
     yp = torch.empty_like(x)
     ys = torch.empty_like(x)
 
-    def run(split, out):
+    def run(split, out, cube_split=True):
         for _ in range(2):
-            fl.c2c_dev(x, 3, out=out, split=split)
+            fl.c2c_dev(x, 3, out=out, split=split, cube_split=cube_split)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            fl.c2c_dev(x, 3, out=out, split=split)
+            fl.c2c_dev(x, 3, out=out, split=split, cube_split=cube_split)
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / steps
 
     ms_p = run(False, yp)
+    ms_l = run(False, ys, cube_split=False)  # FP32 through the line passes the split mode uses
     ms_s = run(True, ys)
     rel = (torch.linalg.vector_norm(ys - yp) / torch.linalg.vector_norm(yp)).item()
     alg = 16.0 * N ** 3
@@ -249,7 +251,8 @@ def split_512(steps):
     del x, yp, ys
     torch.cuda.empty_cache()
     return {"workload": "512^3 C2C forward, U[-1,1) + +-1e-3 spikes on 0.1% of entries, seed 3",
-            "ms_fp32": round(ms_p, 4), "ms_split": round(ms_s, 4),
+            "ms_fp32": round(ms_p, 4), "ms_fp32_line_passes": round(ms_l, 4), "ms_split": round(ms_s, 4),
+            "split_over_fp32_line_passes": round(ms_s / ms_l, 3),
             "gflops_fp32": round(fl_ / (ms_p * 1e-3) / 1e9, 1), "gflops_split": round(fl_ / (ms_s * 1e-3) / 1e9, 1),
             "roofline_frac_fp32": round(alg / (ms_p * 1e-3) / HBM_PEAK, 4),
             "roofline_frac_split": round(alg / (ms_s * 1e-3) / HBM_PEAK, 4),
@@ -499,7 +502,7 @@ def main():
     bitrev = rank == 1
     fusion = {"on": not args.no_plane_fusion, "cube": not args.no_cube_split}
     fusable = rank >= 2  # plane passes / one-pass 4096-point columns where the shape has them
-    cubic = rank == 3 and dims[0] == dims[1] == dims[2] and dims[0] in (128, 256)  # y-split pair
+    cubic = rank == 3 and dims[0] == dims[1] == dims[2] and dims[0] in (128, 256, 512)  # y-split pair
 
     def step():
         fl.c2c_dev(x, rank, inverse=False, out=X, bitrev_1d=bitrev, plane_fusion=fusion["on"],

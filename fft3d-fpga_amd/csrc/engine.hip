@@ -353,9 +353,12 @@ static int noalloc_setup(int dev) {
 static bool is_pow2(size_t x) { return x && !(x & (x - 1)); }
 
 static int split_contig(int prec, int lg, int* lg1, int* lg2) {
-  // L = L1 (strided pass) * L2 (contiguous pass)
-  const int mc = ilog2c(lmax_contig(prec)), ms = ilog2c(lmax_strided(prec));
+  // L = L1 (strided pass with the four-step twiddle) * L2 (contiguous pass);
+  // 2^22 as 1024 x 4096 (the contiguous pass then runs four lines per CU and
+  // three stages: 38.1 against 41.4 us for 512 x 8192)
+  const int mc = ilog2c(lmax_contig(prec)), ms = ilog2c(lmax_strided_fourstep(prec));
   int b = std::min(mc, lg - 1);
+  if (prec == PREC_F32 && lg == 22) b = 12;
   if (lg - b > ms) return -1;
   *lg2 = b;
   *lg1 = lg - b;

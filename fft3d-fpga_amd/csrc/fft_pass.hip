@@ -53,7 +53,9 @@ static bool dispatch_lg(int lg, const PassArgs& a, unsigned long long blocks, hi
       // split mode exists for FP32 lines with at least one radix-16 stage
       constexpr bool split_ok = !SPLIT || (sizeof(R) == 4 && L >= 16 && L <= 4096);
       constexpr bool ok = split_ok &&
-                          (MODE == MODE_CONTIG ? L <= lmax_contig(prec) : L <= lmax_strided(prec)) &&
+                          (MODE == MODE_CONTIG ? L <= lmax_contig(prec)
+                                                : L <= (EPI == EPI_TWIDDLE ? lmax_strided_fourstep(prec)
+                                                                           : lmax_strided(prec))) &&
                           !(MODE == MODE_CONTIG && EPI == EPI_TWIDDLE) &&
                           !(MODE == MODE_STRIDED && EPI == EPI_BITREV);
       if constexpr (ok) {

@@ -53,10 +53,11 @@ constexpr int geom_tpl(int L) { return L / geom_ppt(L); }
 // columns (128-byte row segments; narrower segments measured far slower) and
 // 256 threads up to L = 256; 32 columns (256-byte segments, 1024 threads) at
 // L = 512, where rows lie 4 KiB to 2 MiB apart (measured 7-18% faster than 16
-// on the y and z passes of a 512^3 array; slower at L = 256).
+// on the y and z passes of a 512^3 array; slower at L = 256); 16 columns
+// (1024 threads) at L = 1024, the four-step passes of long 1D axes.
 constexpr int geom_cols(int L, int mode) {
   return mode == MODE_CONTIG ? (geom_tpl(L) >= 256 ? 1 : 256 / geom_tpl(L))
-                             : (L >= 512 ? 32 : L >= 256 ? 16 : 256 / geom_tpl(L));
+                             : (L >= 1024 ? 16 : L >= 512 ? 32 : L >= 256 ? 16 : 256 / geom_tpl(L));
 }
 constexpr int geom_threads(int L, int mode) { return geom_cols(L, mode) * geom_tpl(L); }
 // Split-precision stages borrow LDS per wave: strided 512-point lines keep 16
@@ -70,5 +71,10 @@ constexpr int geom_threads_for(int L, int mode, bool split) { return geom_cols_f
 // four-step into two passes.
 constexpr int lmax_contig(int prec) { return prec == PREC_F32 ? 8192 : 4096; }
 constexpr int lmax_strided(int prec) { return prec == PREC_F32 ? 512 : 256; }
+// The strided (first) pass of a four-step split of a long contiguous axis may
+// hold 1024-point lines (16 columns, 1024 threads, 139 KiB of LDS): 2^22 points
+// as 1024 x 4096 measured 38.1 against 41.4 us for 512 x 8192.  Strided axes
+// of 2D and 3D arrays keep lmax_strided.
+constexpr int lmax_strided_fourstep(int prec) { return prec == PREC_F32 ? 1024 : 256; }
 
 }  // namespace fftamd

@@ -147,9 +147,9 @@ def c2c_dev(x, rank: int, inverse: bool = False, bitrev_1d: bool = False, out=No
     split=True selects the split-precision FP16 matrix-core radix-16 stages;
     plane_fusion=False runs separate line passes instead of the fused ones
     (the plane pass over the two fastest axes, the one-pass 4096-point column
-    transform, the y-split pair of 256^3 blocks); cube_split=False keeps the
-    256 x 256 plane pass and a z pass for 256^3 blocks instead of the y-split
-    pair."""
+    transform, the y-split pairs of 128^3, 256^3 and 512^3 blocks);
+    cube_split=False runs such blocks without the y-split pair (the plane pass
+    and a z pass for 128^3 and 256^3, three line passes for 512^3)."""
     import torch
     if not x.is_cuda:
         raise ValueError("c2c_dev needs a GPU tensor")

@@ -241,7 +241,7 @@ def test_2d_ddr_vs_oracle(gpu_lib, orc, N, inv):
 @pytest.mark.parametrize("N,batch", [(1, 3), (2, 7), (4, 5), (8, 3), (16, 3), (32, 5), (64, 3),
                                      (128, 2), (256, 9), (512, 2), (1024, 3), (2048, 2),
                                      (4096, 2), (8192, 2), (1 << 14, 2), (1 << 16, 1),
-                                     (1 << 20, 1), (1 << 22, 1)])
+                                     (1 << 20, 1), (1 << 22, 1), (1 << 23, 1)])
 @pytest.mark.parametrize("inv", [False, True])
 def test_1d_bitreversed_vs_oracle(gpu_lib, orc, N, batch, inv):
     x = orc.uniform(N * batch, 300 + N)

@@ -268,9 +268,14 @@ def test_plan_shapes_of_headline_configs(fl):
     p, fc = describe(fl, [512, 512, 512], 1, aliased=True)
     assert [(q["src"], q["dst"]) for q in p] == [(1, 2), (2, 1)] and not fc
     assert [q["mode"] for q in describe(fl, [512, 512, 512], 1, flags=8)[0]] == [0, 1, 1]
-    # 1D 2^22 bit-reversed: four-step, strided 512 then contiguous 8192
+    # 1D 2^22 bit-reversed: four-step, strided 1024 (16 columns) then contiguous 4096
     p, fc = describe(fl, [1 << 22], 1, 1)
-    assert [(q["lg"], q["mode"], q["epi"]) for q in p] == [(9, 1, 1), (13, 0, 2)]
+    assert [(q["lg"], q["mode"], q["epi"], q["C"], q["NT"]) for q in p] == [(10, 1, 1, 16, 1024), (12, 0, 2, 1, 256)]
+    # 2^21 keeps 256 x 8192; 2^23 fits as 1024 x 8192
+    assert [q["lg"] for q in describe(fl, [1 << 21], 1, 1)[0]] == [8, 13]
+    assert [q["lg"] for q in describe(fl, [1 << 23], 1, 1)[0]] == [10, 13]
+    # strided axes of 3D arrays keep 512-point lines (1024 as a 32 x 32 four-step pair)
+    assert [q["lg"] for q in describe(fl, [1024, 1024, 1024], 1)[0]] == [10, 5, 5, 5, 5]
     # 2D 4096^2: the 4096-point column pass first (out of place), then the
     # contiguous 4096-point rows in place on the output; no temporary buffer
     p, fc = describe(fl, [4096, 4096], 1)
@@ -837,3 +842,4 @@ def test_c512_pass_b_block_mapping():
     for s in np.unique(seg)[:200]:
         members = b[seg == s]
         assert len(members) == 2 and members[1] - members[0] == 8 and members[0] % 8 == members[1] % 8
+
