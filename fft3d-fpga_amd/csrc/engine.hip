@@ -1576,6 +1576,15 @@ int runtime_init(int dev, bool svm) {
   HIP_TRY(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
   for (auto& e : st.ev) HIP_TRY(hipEventCreate(&e));
   if (noalloc_setup(dev)) return -1;  // the allocation-free entry point's scratch
+  // every kernel file's code object, so that no later call (one under stream
+  // capture included) has one to load
+  preload_pass();
+  preload_cube();
+  preload_c512();
+  preload_xy();
+  preload_col();
+  preload_plane();
+  preload_multi();
   st.dev = dev;
   st.svm = svm;
   st.ready = true;

@@ -245,6 +245,8 @@ bool col4096_supported(long long lx, unsigned long long nouter) {
                                              CL_C * 8ull < (1ull << 31) && nouter > 0;
 }
 
+void preload_col() { preload_kernel(reinterpret_cast<const void*>(&k_fft_col4096<false, false, false>)); }
+
 int launch_fft_col4096(const ColArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (!col4096_supported(a.lx, a.nitems / (a.ngroups ? a.ngroups : 1)) || a.ngroups != (unsigned)(a.lx / CL_C) ||

@@ -396,6 +396,8 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   }
 }
 
+void preload_cube() { preload_kernel(reinterpret_cast<const void*>(&k_fft_cube_a<false, false>)); }
+
 static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsigned long long* blocks) {
   const unsigned long long b = nplanes * per;
   if (b > 0xFFFFFFFFull / CB_NT) return false;  // grid size in work-items is 32-bit (512 threads bounds every kernel here)

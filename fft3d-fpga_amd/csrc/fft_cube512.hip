@@ -217,6 +217,8 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_b(const CubeArgs a) {
   }
 }
 
+void preload_c512() { preload_kernel(reinterpret_cast<const void*>(&k_fft_c512_a<false, false>)); }
+
 int launch_fft_c512_a(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (a.n != C5_L || a.nplanes == 0 || a.w_store || a.tab || !a.w512 || !a.tw) return -1;

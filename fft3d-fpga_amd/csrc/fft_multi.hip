@@ -161,6 +161,8 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
   }
 }
 
+void preload_multi() { preload_kernel(reinterpret_cast<const void*>(&k_multi_reset)); }
+
 int reset_multi_ctl(MultiCtl* ctl, void* stream) {
   FFTAMD_LAUNCH(k_multi_reset, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), ctl);
   return hipGetLastError() == hipSuccess ? 0 : -1;

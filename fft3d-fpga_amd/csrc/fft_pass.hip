@@ -82,6 +82,8 @@ static bool dispatch_prec(int lg, int mode, int epi, const PassArgs& a, unsigned
   return false;
 }
 
+void preload_pass() { preload_kernel(reinterpret_cast<const void*>(&k_fft_pass<float, 256, MODE_CONTIG, EPI_NONE, false, 0>)); }
+
 int launch_fft_pass(int prec, int lg, int mode, int epi, const PassArgs& a, unsigned long long blocks,
                     void* stream, bool split) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

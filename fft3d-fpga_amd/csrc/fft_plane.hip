@@ -255,6 +255,8 @@ static bool dispatch_ly(int ly, int lx, const PlaneArgs& a, hipStream_t s) {
   }
 }
 
+void preload_plane() { preload_kernel(reinterpret_cast<const void*>(&k_fft_plane<64, 64, false, false>)); }
+
 int launch_fft_plane(int ly, int lx, const PlaneArgs& a, void* stream, bool split) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (!plane_shape_supported(ly, lx) || a.nplanes == 0 || a.nplanes > 0xFFFFFFFFull / 1024 ||

@@ -290,6 +290,8 @@ static void launch_xy(const PlaneArgs& a, hipStream_t s, unsigned blocks) {
   else launch_xyi<false>(a, s, blocks);
 }
 
+void preload_xy() { preload_kernel(reinterpret_cast<const void*>(&k_fft_xy256<false, false, false>)); }
+
 int launch_fft_xy256(const PlaneArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   // 4 blocks per plane, planes rounded up to a multiple of 8 (one per XCD)

@@ -12,6 +12,22 @@
 
 namespace fftamd {
 bool& resolve_only_flag();
+
+// One per kernel source file: looks one of its kernels up, which loads the
+// file's code object onto the current device.  The runtime calls them all when
+// it initialises a device, so that no later call -- one under stream capture
+// included -- has a code object to load.
+void preload_pass();
+void preload_cube();
+void preload_c512();
+void preload_xy();
+void preload_col();
+void preload_plane();
+void preload_multi();
+inline void preload_kernel(const void* f) {
+  hipFuncAttributes fa;
+  (void)hipFuncGetAttributes(&fa, f);
+}
 }  // namespace fftamd
 
 #define FFTAMD_LAUNCH(K, G, B, SHM, S, ...)                                       \
