@@ -709,6 +709,7 @@ def test_cube_split_index_maps(n):
     assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+
 # ---- 512^3 blocks: the y-split pair with y = a + 8 b (fft_cube512.hip) --------------------
 
 def test_c512_split_formula():
@@ -722,19 +723,21 @@ def test_c512_split_formula():
     assert np.allclose(X.reshape(-1), np.fft.fft(x), atol=1e-10)  # index k + 64 d
 
 
-def emulate_c512_a(plane, res):
-    """Pass A on one 512 x 512 plane for residue res: thread (c, tau) loads points
-    tau + 16 r of row y = res + 8 c and does the radix-32 stage; round R writes
-    its sub-transforms 16 R .. 16 R + 15 to LDS [(c 16 + k) 17 + tau] and thread
-    (c, tau') runs radix 16 over tau for k1 = 16 R + tau' (ending with kx = k1 +
-    32 m); column half R puts point (row c, column tau + 16 m) at LDS
-    [c 257 + tau + 16 m] and column thread (cc, tq) stores point k = tq + 4 r of
-    column kx = 16 R + (cc & 15) + 32 (cc >> 4) to row 64 res + k.  Returns the
-    64 output rows (NaN where not written)."""
+def emulate_c512_a(plane, res, pidx=0):
+    """Pass A on one 512 x 512 plane (index pidx) for residue res: thread
+    (c, tau) loads points tau + 16 r of row y = res + 8 b, b = (c + 8 (pidx mod
+    8)) mod 64, and does the radix-32 stage; round R writes its sub-transforms
+    16 R .. 16 R + 15 to LDS [(c 16 + k) 17 + tau] and thread (c, tau') runs
+    radix 16 over tau for k1 = 16 R + tau' (ending with kx = k1 + 32 m); column
+    half R puts point (row b, column tau + 16 m) at LDS [b 257 + tau + 16 m] and
+    column thread (cc, tq) stores point k = tq + 4 r of column kx = 16 R +
+    (cc & 15) + 32 (cc >> 4) to row 64 res + k.  Returns the 64 output rows
+    (NaN where not written)."""
     tid = np.arange(1024)
     c, tau = tid >> 4, tid & 15
+    b = (c + 8 * (pidx & 7)) & 63
     r = np.arange(32)
-    v = plane[(res + 8 * c)[:, None], tau[:, None] + 16 * r[None, :]]  # [tid, r]
+    v = plane[(res + 8 * b)[:, None], tau[:, None] + 16 * r[None, :]]  # [tid, r]
     Y = np.fft.fft(v, axis=1)  # Y_tau[k1]
     o = np.empty((2, 1024, 16), complex)
     for R in range(2):
@@ -753,7 +756,7 @@ def emulate_c512_a(plane, res):
     for R in range(2):
         lds = np.full(64 * 257, np.nan + 0j)
         for m in range(16):
-            idx = c * 257 + tau + 16 * m
+            idx = b * 257 + tau + 16 * m
             assert np.isnan(lds[idx]).all(), "column LDS slot written twice"
             lds[idx] = o[R][:, m]
         cols = lds[np.arange(64)[None, :] * 257 + np.arange(256)[:, None]]  # [col, b]
@@ -768,11 +771,11 @@ def emulate_c512_a(plane, res):
 
 def test_c512_pass_a_maps():
     rng = np.random.default_rng(21)
-    for _ in range(2):
+    for pidx in (0, 5):  # row rotation 0 and 40
         plane = rng.standard_normal((512, 512)) + 1j * rng.standard_normal((512, 512))
         full = np.full((512, 512), np.nan + 0j)
         for res in range(8):
-            full[64 * res:64 * res + 64] = emulate_c512_a(plane, res)
+            full[64 * res:64 * res + 64] = emulate_c512_a(plane, res, pidx)
         assert not np.isnan(full).any()
         # row 64 a + k holds T_a[k] = sum_b FFT_x(x[a + 8 b]) W64^{b k}
         ref = np.fft.fft(np.fft.fft(plane, axis=1).reshape(64, 8, 512), axis=0)  # [k, a, kx]
@@ -829,13 +832,15 @@ def test_c512_pass_b_maps(k):
 
 
 def test_c512_pass_b_block_mapping():
-    """blockIdx -> (k, column group): every tile of a block once, and the two
-    8-column groups of a 128-byte segment on one XCD (blocks b and b + 8)."""
+    """blockIdx -> (k, column group): every tile of a block once, the two
+    8-column groups of a 128-byte segment on one XCD (blocks b and b + 8), and
+    the first 256 workgroups (one per CU) spread over all 64 rows k."""
     b = np.arange(2 * 4096)
     P = (b >> 4) * 8 + (b & 7)
-    xg = 2 * (P & 31) + ((b >> 3) & 1)
-    k = (P >> 5) & 63
+    k = P & 63
+    xg = 2 * ((P >> 6) & 31) + ((b >> 3) & 1)
     blk = P >> 11
+    assert len(np.unique(k[:256])) == 64
     key = (blk * 64 + k) * 64 + xg
     assert np.array_equal(np.sort(key), np.arange(2 * 4096))
     seg = (blk * 64 + k) * 32 + xg // 2
