@@ -163,19 +163,21 @@ __device__ __forceinline__ void col_quarter(const ColArgs& a, unsigned long long
   // over n2 (rows n1 + 256 n2), outputs k1 = Q + 4u; u = 0, 1 go to LDS as
   // lines 16 u + c16, u = 2, 3 stay in registers
   cpx<float> held[CL_PER][2];
+  const int rot = (int)(grp & 7u);
 #pragma unroll
   for (int i = 0; i < CL_PER; ++i) {
+    const int ir = (i + rot) & 7;
     cpx<float> v[16];
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) {
-      const unsigned soff = (unsigned)(CL_NG * i + CL_N1 * n2) * lx8;
+      const unsigned soff = (unsigned)(CL_NG * ir + CL_N1 * n2) * lx8;
       cpx<float> s = __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(rin, voff, soff, 0));
       if (WL) s = cscale(s, __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rwl, voff / 2u, soff / 2u, 0)));
       v[n2] = INV ? cswap(s) : s;
     }
     cpx<float> out4[4];
     col_residue<Q>(v, out4);
-    const int n1 = gi + CL_NG * i;
+    const int n1 = gi + CL_NG * ir;
     lds[ColGeom::idx(c16, n1)] = out4[0];
     lds[ColGeom::idx(CL_C + c16, n1)] = out4[1];
     held[i][0] = out4[2];
@@ -189,7 +191,7 @@ __device__ __forceinline__ void col_quarter(const ColArgs& a, unsigned long long
   __syncthreads();  // the last stage has read the buffer
 #pragma unroll
   for (int i = 0; i < CL_PER; ++i) {
-    const int n1 = gi + CL_NG * i;
+    const int n1 = gi + CL_NG * ((i + rot) & 7);
     lds[ColGeom::idx(c16, n1)] = held[i][0];
     lds[ColGeom::idx(CL_C + c16, n1)] = held[i][1];
   }

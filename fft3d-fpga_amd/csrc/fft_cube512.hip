@@ -139,14 +139,16 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_b(const CubeArgs a) {
   __shared__ cpx<float> w512[C5_L];
   // the two 8-column groups of one 128-byte row segment are blocks b and
   // b + 8: consecutive tiles of one XCD (workgroups go to XCDs round-robin).
-  // k varies fastest over the pairs, so the tiles in flight together cover all
-  // 64 rows k (address bits 12-17) instead of four; every tile's own addresses
-  // differ only in multiples of 256 KiB (measured on this access pattern alone:
-  // 510 against 563 us per 1 GiB)
+  // Every tile's own addresses differ only in multiples of 256 KiB, so the
+  // pair index P interleaves k and the column pair: bits 0-3 and 7-8 of P
+  // give k, bits 4-6 and 9-10 the pair, and the 128 pairs in flight together
+  // (one workgroup per CU) cover 16 rows k x 8 column pairs (address bits
+  // 8-15) instead of 4 rows x all 32 pairs (measured on this access pattern
+  // alone: 499 against 563 us per 1 GiB; all 64 rows x 2 pairs: 510-520 us)
   const unsigned b = blockIdx.x;
   const unsigned P = (b >> 4) * 8 + (b & 7);  // pair index
-  const int k = (int)(P & 63);                                       // rows k + 64 a
-  const int xg = 2 * (int)((P >> 6) & 31) + (int)((b >> 3) & 1);  // group of 8 columns
+  const int k = (int)((P & 15u) | (((P >> 7) & 3u) << 4));                             // rows k + 64 a
+  const int xg = 2 * (int)(((P >> 4) & 7u) | (((P >> 9) & 3u) << 3)) + (int)((b >> 3) & 1);  // 8 columns
   const unsigned long long blk = P >> 11;
   if (blk >= a.nplanes / C5_L) return;  // uniform over the workgroup
   const long long base = (long long)blk * C5_BLOCK + (long long)k * C5_L + 8 * xg;

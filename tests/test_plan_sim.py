@@ -834,13 +834,14 @@ def test_c512_pass_b_maps(k):
 def test_c512_pass_b_block_mapping():
     """blockIdx -> (k, column group): every tile of a block once, the two
     8-column groups of a 128-byte segment on one XCD (blocks b and b + 8), and
-    the first 256 workgroups (one per CU) spread over all 64 rows k."""
+    the first 256 workgroups (one per CU) spread over 16 rows k x 16 column
+    groups."""
     b = np.arange(2 * 4096)
     P = (b >> 4) * 8 + (b & 7)
-    k = P & 63
-    xg = 2 * ((P >> 6) & 31) + ((b >> 3) & 1)
+    k = (P & 15) | (((P >> 7) & 3) << 4)
+    xg = 2 * (((P >> 4) & 7) | (((P >> 9) & 3) << 3)) + ((b >> 3) & 1)
     blk = P >> 11
-    assert len(np.unique(k[:256])) == 64
+    assert len(np.unique(k[:256])) == 16 and len(np.unique(xg[:256])) == 16
     key = (blk * 64 + k) * 64 + xg
     assert np.array_equal(np.sort(key), np.arange(2 * 4096))
     seg = (blk * 64 + k) * 32 + xg // 2
