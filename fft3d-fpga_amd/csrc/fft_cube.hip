@@ -155,9 +155,12 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
 template <bool INV, bool WS>
 __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
   __shared__ cpx<float> buf[CbZ::LDS_ELEMS];
+  // k varies fastest over blockIdx: the tiles in flight together then cover
+  // all 64 rows k (measured on this access pattern alone, in place after pass
+  // A: 50.3 against 53.4 us with the column group fastest)
   const unsigned b = blockIdx.x;
-  const int xg = b & 15;               // group of 16 columns
-  const int k = (b >> 4) & 63;         // rows k + 64 a
+  const int k = b & 63;                // rows k + 64 a
+  const int xg = (b >> 6) & 15;        // group of 16 columns
   const unsigned long long blk = b >> 10;
   if (blk >= a.nplanes / CB_L) return;  // uniform over the workgroup
   const long long base = (long long)blk * CB_BLOCK + (long long)k * CB_L + 16 * xg;
