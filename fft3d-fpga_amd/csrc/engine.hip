@@ -1406,9 +1406,25 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
     }
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  // the events are released and the plan's schedule restored on every exit
+  struct Ev {
+    hipEvent_t e = nullptr;
+    ~Ev() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } e0, e1;
+  const int prev_mode = mp->mode;
+  const unsigned prev_grid = mp->grid, prev_chunk = mp->chunk;
+  auto fail = [&]() {
+    mp->mode = prev_mode;
+    mp->grid = prev_grid;
+    mp->chunk = prev_chunk;
+    return -1;
+  };
+  if (hipEventCreate(&e0.e) != hipSuccess || hipEventCreate(&e1.e) != hipSuccess) {
+    set_error("tuner event creation failed");
+    return fail();
+  }
   double best = 1e30;
   int bmode = 0;
   unsigned bpc = per, bch = 4;
@@ -1421,17 +1437,18 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
         mp->mode = mode;
         mp->grid = pc * ncu;
         mp->chunk = ch;
-        if (multi_exec(mp, s) || multi_check(mp, s)) {  // warm-up and error check
-          (void)hipEventDestroy(e0);
-          (void)hipEventDestroy(e1);
-          return -1;
+        if (multi_exec(mp, s) || multi_check(mp, s)) return fail();  // warm-up and error check
+        if (hipEventRecord(e0.e, s) != hipSuccess) {
+          set_error("tuner event record failed: %s", hipGetErrorString(hipGetLastError()));
+          return fail();
         }
-        HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < std::max(1, reps); ++r)
-          if (multi_exec(mp, s)) return -1;
-        HIP_TRY(hipEventRecord(e1, s));
-        HIP_TRY(hipEventSynchronize(e1));
-        const double us = 1e3 * ms_between(e0, e1) / std::max(1, reps);
+          if (multi_exec(mp, s)) return fail();
+        if (hipEventRecord(e1.e, s) != hipSuccess || hipEventSynchronize(e1.e) != hipSuccess) {
+          set_error("tuner event record failed: %s", hipGetErrorString(hipGetLastError()));
+          return fail();
+        }
+        const double us = 1e3 * ms_between(e0.e, e1.e) / std::max(1, reps);
         if (us < best) {
           best = us;
           bmode = mode;
@@ -1439,9 +1456,7 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
           bch = ch;
         }
       }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (multi_check(mp, s)) return -1;
+  if (multi_check(mp, s)) return fail();
   mp->mode = bmode;
   mp->grid = bpc * ncu;
   mp->chunk = bch;
