@@ -709,6 +709,17 @@ def test_cube_split_index_maps(n):
     assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+def test_cube256_pass_b_block_mapping():
+    """256^3 pass B, blockIdx -> (block, k, column group) with k fastest: every
+    tile of two blocks once, and the 512 workgroups in flight together (two per
+    CU) cover all 64 rows k (and 8 of the 16 column groups) of the first block."""
+    b = np.arange(2 * 1024)
+    k, xg, blk = b & 63, (b >> 6) & 15, b >> 10
+    assert np.array_equal(np.sort((blk * 64 + k) * 16 + xg), b)
+    assert len(np.unique(k[:512])) == 64 and len(np.unique(xg[:512])) == 8
+    assert len(np.unique(xg[:1024])) == 16
+
+
 
 # ---- 512^3 blocks: the y-split pair with y = a + 8 b (fft_cube512.hip) --------------------
 
