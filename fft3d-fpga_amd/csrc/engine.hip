@@ -409,7 +409,7 @@ static int build_plan(int rank, const size_t* dims, size_t batch, unsigned flags
   // stage and z (in place)
   int top = rank - 1;
   const long long CN = rank >= 3 ? (long long)dims[rank - 1] : 0;
-  if (rank >= 3 && prec == PREC_F32 && !plan.split && !(flags & (FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT)) &&
+  if (rank >= 3 && prec == PREC_F32 && !(flags & (FLAG_NO_PLANE_FUSION | FLAG_NO_CUBE_SPLIT)) &&
       (CN == 512 || CN == 256 || CN == 128) && (long long)dims[rank - 2] == CN && (long long)dims[rank - 3] == CN) {
     unsigned long long O = batch;
     for (int d = 0; d < rank - 3; ++d) O *= dims[d];
@@ -800,6 +800,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       ca.w_store = !A && pi + 1 == plan.passes.size() ? w_store : nullptr;
       ca.w_mask = (unsigned long long)plan.per - 1ULL;
       ca.inverse = inverse ? 1 : 0;
+      ca.split = plan.split ? 1 : 0;
       Prof& pf = P();
       const bool prof = pf.on && !resolve && pf.used + 2 <= pf.pool.size();
       if (prof) HIP_TRY(hipEventRecord(pf.pool[pf.used], stream));
@@ -807,7 +808,7 @@ static int run_plan(Plan& plan, const void* din, void* dout, bool inverse, hipSt
       if (prof) {
         HIP_TRY(hipEventRecord(pf.pool[pf.used + 1], stream));
         char lab[64];
-        snprintf(lab, sizeof(lab), "fft_cube_%s_f32_%d", A ? "a" : "b", ca.n);
+        snprintf(lab, sizeof(lab), "fft_cube_%s_f32%s_%d", A ? "a" : "b", plan.split ? "_split" : "", ca.n);
         pf.recs.push_back({lab, pf.pool[pf.used], pf.pool[pf.used + 1]});
         pf.used += 2;
       }
@@ -1139,7 +1140,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
       g.tab_off = tab.size() / 2;
       // 128^3 transforms run through the y-split pair, whose first pass runs
       // out of place: only when no transform's input and output overlap
-      g.cube = !mp->split && g.ppt == 128 && g.ly == 128 && g.lx == 128;
+      g.cube = g.ppt == 128 && g.ly == 128 && g.lx == 128;
       for (unsigned i : it->second) {
         const unsigned long long bytes = 8ull * 128 * 128 * 128;
         const unsigned long long ib = (unsigned long long)(uintptr_t)din + 8ull * in_off[i];
@@ -1214,6 +1215,7 @@ static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stre
     if (get_stockham(PREC_F32, 128, &ca.tw)) return -1;
     if (get_direct(256, &ca.w256)) return -1;
     ca.inverse = mp->inverse ? 1 : 0;
+    ca.split = mp->split ? 1 : 0;
     ca.in = mp->din;
     ca.out = mp->dout;
     ca.tab = mp->d_tab + 2 * g.tab_off;

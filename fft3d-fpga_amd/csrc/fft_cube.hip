@@ -48,11 +48,12 @@ static_assert(CaRow::NT == CB_NT && CaCol::NT == CB_NT && CbZ::NT == CB_NT, "512
 static_assert(sched_np(CB_L) == 2 && sched_np(64) == 2, "two stages per line");
 constexpr int CA_LDS = CaRow::LDS_ELEMS > CaCol::LDS_ELEMS ? CaRow::LDS_ELEMS : CaCol::LDS_ELEMS;
 
-// Two workgroups per CU: at most 128 VGPRs (4 waves per SIMD), <= 68 KiB LDS each.
-template <bool INV, bool WL>
+// Two workgroups per CU: at most 128 VGPRs (4 waves per SIMD), <= 74 KiB LDS each.
+template <bool INV, bool WL, bool SPLIT>
 __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
-  __shared__ cpx<float> buf[CA_LDS];
+  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(CA_LDS, CB_NT) : CA_LDS];
   __shared__ cpx<float> tws[16 * 15];  // row stage-1 twiddles W256^{k r} at [15 k + r - 1]
+  __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   const unsigned long long plane = blockIdx.x >> 2;
   const int res = blockIdx.x & 3;  // residue class a of the input rows
   if (plane >= a.nplanes) return;  // uniform over the workgroup
@@ -62,6 +63,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
   const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
   const int tid = threadIdx.x;
   if (tid < 16 * 15) tws[tid] = wdir[(tid / 15) * (tid % 15 + 1)];  // read after the first barrier
+  if constexpr (SPLIT) split_atab_fill(atab);
 
   // ---- x: rows b = 32 h + c (y = res + 4 b); thread (c, tau) ends with
   // point kx = tau + 16 r of its row in r0[r] (h = 0) and r1[r] (h = 1)
@@ -85,7 +87,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     cpx<float>* v = h ? r1 : r0;
-    stage_compute<float, CB_L, MODE_CONTIG, 0, false, 32>(v, tau, nullptr);  // radix 16, no twiddles
+    stage_compute<float, CB_L, MODE_CONTIG, 0, SPLIT, 32, SPLIT>(v, tau, nullptr, buf, atab);  // radix 16, no twiddles
     if (h) __syncthreads();  // every thread has read the first half's stage-1 inputs
     stage_store_lds<float, CB_L, MODE_CONTIG, 0, 32>(v, buf, c, tau);
     __syncthreads();
@@ -94,7 +96,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
     // from the LDS table (lanes of one tau read one address)
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * tau + r - 1]);
-    dft16(v);
+    wg_dft16<SPLIT, true>(v, buf, atab);  // the buffer has been read
   }
 
   // ---- y over b: columns kx = 128 hc + cc, 64-point lines; thread (cc, tq)
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
     }
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[4 * tq * r]);
-    dft16(v);
+    wg_dft16<SPLIT, true>(v, buf, atab);  // the buffer has been read
     // column DFT point k = tq + 4 r to row 64 res + k (pass B applies W256^{res k})
 #pragma unroll
     for (int r = 0; r < 16; ++r)
@@ -152,9 +154,10 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
   }
 }
 
-template <bool INV, bool WS>
+template <bool INV, bool WS, bool SPLIT>
 __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
-  __shared__ cpx<float> buf[CbZ::LDS_ELEMS];
+  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(CbZ::LDS_ELEMS, CB_NT) : CbZ::LDS_ELEMS];
+  __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   // k varies fastest over blockIdx: the tiles in flight together then cover
   // all 64 rows k (measured on this access pattern alone, in place after pass
   // A: 50.3 against 53.4 us with the column group fastest)
@@ -168,6 +171,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
   cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + base;
   const cpx<float>* __restrict__ tw256 = reinterpret_cast<const cpx<float>*>(a.tw);
   const int tid = threadIdx.x;
+  if constexpr (SPLIT) split_atab_fill(atab);
 
   // ---- y over a: thread (x, zs) holds planes z = zs + 32 j; t[j][d] = X[k + 64 d].
   // Sixteen adjacent lanes read one 128-byte row segment.
@@ -218,7 +222,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       v[q] = cadd(buf[CbZ::idx(cz, q + 32 * m)], cmul(buf[CbZ::idx(cz, q + 16 + 32 * m)], w16));
-    stage_compute<float, CB_L, MODE_STRIDED, 1, false, 32>(v, tz, tw256);  // W256^{k0 zq}, DFT over zq
+    stage_compute<float, CB_L, MODE_STRIDED, 1, SPLIT, 32, SPLIT>(v, tz, tw256, buf, atab);  // W256^{k0 zq}, DFT over zq
     // point kz = tz + 16 r of row k + 64 d, column 16 xg + (cz & 15)
     const long long ob = (long long)(64 * (2 * e + (cz >> 4))) * CB_L + (cz & 15);
 #pragma unroll
@@ -249,9 +253,9 @@ static_assert(C1Row::NT == C1_NT && C1Col::NT == C1_NT && C1Z::NT == C1B_NT, "25
 static_assert(sched_np(C1_L) == 2 && sched_np(32) == 2, "two stages per line");
 constexpr int C1A_LDS = C1Row::LDS_ELEMS > C1Col::LDS_ELEMS ? C1Row::LDS_ELEMS : C1Col::LDS_ELEMS;
 
-template <bool INV, bool WL>
+template <bool INV, bool WL, bool SPLIT>
 __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
-  __shared__ cpx<float> buf[C1A_LDS];
+  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C1A_LDS, C1_NT) : C1A_LDS];
   __shared__ cpx<float> tws[8 * 15];  // row stage-1 twiddles W128^{k r} at [15 k + r - 1]
   const unsigned long long plane = blockIdx.x >> 2;
   const int res = blockIdx.x & 3;  // residue class a of the input rows
@@ -296,7 +300,7 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
   // the LDS table (lanes of one tau read one address)
 #pragma unroll
   for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * tau + r - 1]);
-  dft16(v);
+  wg_dft16<SPLIT>(v, buf);  // the buffer has been read
 
   // ---- y over b: 128 columns of 32 points; thread (cc, tq) ends with point
   // k = tq + 2 r of its column
@@ -315,7 +319,7 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
   // radix-16 stage over sub-transforms of span 2: twiddles W32^{tq r} = W256^{8 tq r}
 #pragma unroll
   for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[8 * tq * r]);
-  dft16(v);
+  wg_dft16<SPLIT>(v, buf);  // the buffer has been read
   // column DFT point k = tq + 2 r to row 32 res + k (pass B applies W128^{res k})
 #pragma unroll
   for (int r = 0; r < 16; ++r) out[(long long)(32 * res + tq + 2 * r) * C1_L + cc] = v[r];
@@ -324,9 +328,10 @@ __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
 // Pass B of 128^3 blocks: 32 adjacent columns (256-byte row segments, as the
 // 128-point z line pass; 16 columns measured 102.6 against 98.9 us) x 4 rows
 // x 128 planes (128 KiB) with 512 threads of 32 points, two workgroups per CU.
-template <bool INV, bool WS>
+template <bool INV, bool WS, bool SPLIT>
 __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
-  __shared__ cpx<float> buf[C1Z::LDS_ELEMS];
+  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C1Z::LDS_ELEMS, C1B_NT) : C1Z::LDS_ELEMS];
+  __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   const unsigned b = blockIdx.x;
   const int xg = b & 3;                // group of 32 columns
   const int k = (b >> 2) & 31;         // rows k + 32 a
@@ -340,6 +345,7 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   cpx<float>* out = reinterpret_cast<cpx<float>*>(a.out) + obase;
   const cpx<float>* __restrict__ wdir = reinterpret_cast<const cpx<float>*>(a.w256);
   const int tid = threadIdx.x;
+  if constexpr (SPLIT) split_atab_fill(atab);
 
   // ---- y over a: thread (x, zs) holds planes z = zs + 16 j; t[j][d] = X[k + 32 d]
   const int x = tid & 31, zs = tid >> 5;
@@ -386,7 +392,7 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
     // W256^{2 tz r}, uniform over the wave (scalar loads)
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[2 * tz * r]);
-    dft16(v);
+    wg_dft16<SPLIT, true>(v, buf, atab);  // the buffer has been read
     // point kz = tz + 8 r of row k + 32 d, column 32 xg + (cz & 31)
     const long long ob = (long long)(32 * (2 * e + (cz >> 5))) * C1_L + (cz & 31);
 #pragma unroll
@@ -399,7 +405,7 @@ __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
   }
 }
 
-void preload_cube() { preload_kernel(reinterpret_cast<const void*>(&k_fft_cube_a<false, false>)); }
+void preload_cube() { preload_kernel(reinterpret_cast<const void*>(&k_fft_cube_a<false, false, false>)); }
 
 static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsigned long long* blocks) {
   const unsigned long long b = nplanes * per;
@@ -408,31 +414,34 @@ static bool cube_grid(unsigned long long nplanes, unsigned long long per, unsign
   return true;
 }
 
+// KER<inverse, weights, split>; the split-precision variants take no weights
+#define CUBE_DISPATCH(KER, W, G, T, S, A)                                  \
+  do {                                                                     \
+    if ((A).split) {                                                       \
+      if ((A).inverse) FFTAMD_LAUNCH((KER<true, false, true>), G, T, 0, S, A); \
+      else FFTAMD_LAUNCH((KER<false, false, true>), G, T, 0, S, A);        \
+    } else if ((A).inverse) {                                              \
+      if (W) FFTAMD_LAUNCH((KER<true, true, false>), G, T, 0, S, A);       \
+      else FFTAMD_LAUNCH((KER<true, false, false>), G, T, 0, S, A);        \
+    } else {                                                               \
+      if (W) FFTAMD_LAUNCH((KER<false, true, false>), G, T, 0, S, A);      \
+      else FFTAMD_LAUNCH((KER<false, false, false>), G, T, 0, S, A);       \
+    }                                                                      \
+  } while (0)
+
 int launch_fft_cube_a(const CubeArgs& a, void* stream) {
   if (a.n == 512) return launch_fft_c512_a(a, stream);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
   if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || !cube_grid(a.nplanes, 4, &blocks) || a.w_store ||
-      (a.tab && (a.n != C1_L || a.w_load)))
+      (a.tab && (a.n != C1_L || a.w_load)) || (a.split && a.w_load))
     return -1;
   if (a.n == C1_L) {
     const dim3 g((unsigned)blocks), t(C1_NT);
-    if (a.inverse) {
-      if (a.w_load) FFTAMD_LAUNCH((k_fft_cube128_a<true, true>), g, t, 0, s, a);
-      else FFTAMD_LAUNCH((k_fft_cube128_a<true, false>), g, t, 0, s, a);
-    } else {
-      if (a.w_load) FFTAMD_LAUNCH((k_fft_cube128_a<false, true>), g, t, 0, s, a);
-      else FFTAMD_LAUNCH((k_fft_cube128_a<false, false>), g, t, 0, s, a);
-    }
-    return hipGetLastError() == hipSuccess ? 0 : -2;
-  }
-  const dim3 g((unsigned)blocks), t(CB_NT);
-  if (a.inverse) {
-    if (a.w_load) FFTAMD_LAUNCH((k_fft_cube_a<true, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_cube_a<true, false>), g, t, 0, s, a);
+    CUBE_DISPATCH(k_fft_cube128_a, a.w_load, g, t, s, a);
   } else {
-    if (a.w_load) FFTAMD_LAUNCH((k_fft_cube_a<false, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_cube_a<false, false>), g, t, 0, s, a);
+    const dim3 g((unsigned)blocks), t(CB_NT);
+    CUBE_DISPATCH(k_fft_cube_a, a.w_load, g, t, s, a);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -442,28 +451,16 @@ int launch_fft_cube_b(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   unsigned long long blocks;
   if ((a.n != CB_L && a.n != C1_L) || a.nplanes == 0 || a.nplanes % a.n || a.w_load ||
-      (a.tab && (a.n != C1_L || a.w_store)))
+      (a.tab && (a.n != C1_L || a.w_store)) || (a.split && a.w_store))
     return -1;
   if (a.n == C1_L) {
     if (!cube_grid(a.nplanes / C1_L, 128, &blocks)) return -1;
     const dim3 g((unsigned)blocks), t(C1B_NT);
-    if (a.inverse) {
-      if (a.w_store) FFTAMD_LAUNCH((k_fft_cube128_b<true, true>), g, t, 0, s, a);
-      else FFTAMD_LAUNCH((k_fft_cube128_b<true, false>), g, t, 0, s, a);
-    } else {
-      if (a.w_store) FFTAMD_LAUNCH((k_fft_cube128_b<false, true>), g, t, 0, s, a);
-      else FFTAMD_LAUNCH((k_fft_cube128_b<false, false>), g, t, 0, s, a);
-    }
-    return hipGetLastError() == hipSuccess ? 0 : -2;
-  }
-  if (!cube_grid(a.nplanes / CB_L, 1024, &blocks)) return -1;
-  const dim3 g((unsigned)blocks), t(CB_NT);
-  if (a.inverse) {
-    if (a.w_store) FFTAMD_LAUNCH((k_fft_cube_b<true, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_cube_b<true, false>), g, t, 0, s, a);
+    CUBE_DISPATCH(k_fft_cube128_b, a.w_store, g, t, s, a);
   } else {
-    if (a.w_store) FFTAMD_LAUNCH((k_fft_cube_b<false, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_cube_b<false, false>), g, t, 0, s, a);
+    if (!cube_grid(a.nplanes / CB_L, 1024, &blocks)) return -1;
+    const dim3 g((unsigned)blocks), t(CB_NT);
+    CUBE_DISPATCH(k_fft_cube_b, a.w_store, g, t, s, a);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

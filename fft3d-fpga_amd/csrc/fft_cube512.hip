@@ -54,11 +54,12 @@ static_assert(sched_np(64) == 2 && sched_radix(64, 0) == 4, "64 = 4 x 16");
 static_assert(sched_np(128) == 2 && sched_radix(128, 0) == 8, "128 = 8 x 16");
 constexpr int C5A_LDS = C5X_LDS > C5Col::LDS_ELEMS ? C5X_LDS : C5Col::LDS_ELEMS;
 
-// One workgroup per CU: 1024 threads (128 VGPRs each), <= 144 KiB of LDS.
-template <bool INV, bool WL>
+// One workgroup per CU: 1024 threads (128 VGPRs each), <= 148 KiB of LDS.
+template <bool INV, bool WL, bool SPLIT>
 __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
-  __shared__ cpx<float> buf[C5A_LDS];
+  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C5A_LDS, C5_NT) : C5A_LDS];
   __shared__ cpx<float> w512[C5_L];  // W512^m
+  __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   const unsigned long long plane = blockIdx.x >> 3;
   const int res = blockIdx.x & 7;  // residue class a of the input rows
   if (plane >= a.nplanes) return;  // uniform over the workgroup
@@ -69,6 +70,7 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
   const cpx<float>* __restrict__ tw64 = reinterpret_cast<const cpx<float>*>(a.tw);
   const int tid = threadIdx.x;
   if (tid < C5_L) w512[tid] = wdir[tid];  // read after the first barrier
+  if constexpr (SPLIT) split_atab_fill(atab);
 
   // ---- x: thread (c, tau) transforms row b (y = res + 8 b); stage 0 is the
   // radix-32 DFT of points tau + 16 r straight from HBM.  b = c rotated by
@@ -105,7 +107,7 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
     for (int t = 0; t < 16; ++t) y[t] = buf[(c * 16 + tau) * C5X_K + t];
 #pragma unroll
     for (int t = 1; t < 16; ++t) y[t] = cmul(y[t], w512[(k1 * t) & (C5_L - 1)]);
-    dft16(y);
+    wg_dft16<SPLIT, true>(y, buf, atab);  // the buffer has been read
   }
 
   // ---- y over b: half R holds columns kx = 16 R + tau + 32 m (column index
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
     stage_store_lds<float, 64, MODE_STRIDED, 0, 256>(u, buf, cc, tq);
     __syncthreads();
     stage_load_lds<float, 64, MODE_STRIDED, 1, 256>(u, buf, cc, tq);
-    stage_compute<float, 64, MODE_STRIDED, 1, false, 256>(u, tq, tw64);  // radix 16, W64 twiddles
+    stage_compute<float, 64, MODE_STRIDED, 1, SPLIT, 256, SPLIT>(u, tq, tw64, buf, atab);  // radix 16, W64 twiddles
     // column DFT point k = tq + 4 r to row 64 res + k (pass B applies W512^{res k})
     const int kx = 16 * R + (cc & 15) + 32 * (cc >> 4);
 #pragma unroll
@@ -133,10 +135,11 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
   }
 }
 
-template <bool INV, bool WS>
+template <bool INV, bool WS, bool SPLIT>
 __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_b(const CubeArgs a) {
-  __shared__ cpx<float> buf[C5Z::LDS_ELEMS];
+  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C5Z::LDS_ELEMS, C5_NT) : C5Z::LDS_ELEMS];
   __shared__ cpx<float> w512[C5_L];
+  __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   // the two 8-column groups of one 128-byte row segment are blocks b and
   // b + 8: consecutive tiles of one XCD (workgroups go to XCDs round-robin).
   // Every tile's own addresses differ only in multiples of 256 KiB, so the
@@ -158,6 +161,7 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_b(const CubeArgs a) {
   const cpx<float>* __restrict__ tw128 = reinterpret_cast<const cpx<float>*>(a.tw);
   const int tid = threadIdx.x;
   if (tid < C5_L) w512[tid] = wdir[tid];  // read after the first barrier
+  if constexpr (SPLIT) split_atab_fill(atab);
 
   // ---- y over a: thread (x, zs) holds planes z = zs + 128 j; t[j][d] = X[k + 64 d]
   const int x = tid & 7, zs = tid >> 3;
@@ -215,7 +219,7 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_b(const CubeArgs a) {
     stage_store_lds<float, 128, MODE_STRIDED, 0, 128>(v, buf, cz, tz);
     __syncthreads();
     stage_load_lds<float, 128, MODE_STRIDED, 1, 128>(v, buf, cz, tz);
-    stage_compute<float, 128, MODE_STRIDED, 1, false, 128>(v, tz, tw128);  // radix 16, W128 twiddles
+    stage_compute<float, 128, MODE_STRIDED, 1, SPLIT, 128, SPLIT>(v, tz, tw128, buf, atab);  // radix 16, W128 twiddles
     const int xx = cz & 7, kq = (cz >> 3) & 3, dd = cz >> 5;
     const long long ob = (long long)(64 * (4 * hh + dd)) * C5_L + xx;  // row k + 64 d
 #pragma unroll
@@ -228,37 +232,42 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_b(const CubeArgs a) {
   }
 }
 
-void preload_c512() { preload_kernel(reinterpret_cast<const void*>(&k_fft_c512_a<false, false>)); }
+void preload_c512() { preload_kernel(reinterpret_cast<const void*>(&k_fft_c512_a<false, false, false>)); }
+
+// KER<inverse, weights, split>; the split-precision variants take no weights
+#define C512_DISPATCH(KER, W, G, T, S, A)                                  \
+  do {                                                                     \
+    if ((A).split) {                                                       \
+      if ((A).inverse) FFTAMD_LAUNCH((KER<true, false, true>), G, T, 0, S, A); \
+      else FFTAMD_LAUNCH((KER<false, false, true>), G, T, 0, S, A);        \
+    } else if ((A).inverse) {                                              \
+      if (W) FFTAMD_LAUNCH((KER<true, true, false>), G, T, 0, S, A);       \
+      else FFTAMD_LAUNCH((KER<true, false, false>), G, T, 0, S, A);        \
+    } else {                                                               \
+      if (W) FFTAMD_LAUNCH((KER<false, true, false>), G, T, 0, S, A);      \
+      else FFTAMD_LAUNCH((KER<false, false, false>), G, T, 0, S, A);       \
+    }                                                                      \
+  } while (0)
 
 int launch_fft_c512_a(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (a.n != C5_L || a.nplanes == 0 || a.w_store || a.tab || !a.w512 || !a.tw) return -1;
+  if (a.n != C5_L || a.nplanes == 0 || a.w_store || a.tab || !a.w512 || !a.tw || (a.split && a.w_load)) return -1;
   const unsigned long long blocks = a.nplanes * 8ull;
   if (blocks > 0xFFFFFFFFull / C5_NT) return -1;  // grid size in work-items is 32-bit
   const dim3 g((unsigned)blocks), t(C5_NT);
-  if (a.inverse) {
-    if (a.w_load) FFTAMD_LAUNCH((k_fft_c512_a<true, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_c512_a<true, false>), g, t, 0, s, a);
-  } else {
-    if (a.w_load) FFTAMD_LAUNCH((k_fft_c512_a<false, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_c512_a<false, false>), g, t, 0, s, a);
-  }
+  C512_DISPATCH(k_fft_c512_a, a.w_load, g, t, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int launch_fft_c512_b(const CubeArgs& a, void* stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (a.n != C5_L || a.nplanes == 0 || a.nplanes % C5_L || a.w_load || a.tab || !a.w512 || !a.tw) return -1;
+  if (a.n != C5_L || a.nplanes == 0 || a.nplanes % C5_L || a.w_load || a.tab || !a.w512 || !a.tw ||
+      (a.split && a.w_store))
+    return -1;
   const unsigned long long blocks = (a.nplanes / C5_L) * 4096ull;  // 64 k x 64 column groups per block
   if (blocks > 0xFFFFFFFFull / C5_NT) return -1;
   const dim3 g((unsigned)blocks), t(C5_NT);
-  if (a.inverse) {
-    if (a.w_store) FFTAMD_LAUNCH((k_fft_c512_b<true, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_c512_b<true, false>), g, t, 0, s, a);
-  } else {
-    if (a.w_store) FFTAMD_LAUNCH((k_fft_c512_b<false, true>), g, t, 0, s, a);
-    else FFTAMD_LAUNCH((k_fft_c512_b<false, false>), g, t, 0, s, a);
-  }
+  C512_DISPATCH(k_fft_c512_b, a.w_store, g, t, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

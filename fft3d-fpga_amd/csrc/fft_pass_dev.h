@@ -42,9 +42,10 @@ struct Geom {
   }
 };
 
-template <typename R, int L, int MODE, int S, bool SPLIT = false, int CC = geom_cols(L, MODE)>
+// TAB: split stages read their constant operands from the LDS table `atab`
+template <typename R, int L, int MODE, int S, bool SPLIT = false, int CC = geom_cols(L, MODE), bool TAB = false>
 __device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* __restrict__ tw,
-                                              cpx<R>* lds = nullptr) {
+                                              cpx<R>* lds = nullptr, const split_half8* atab = nullptr) {
   using G = Geom<L, MODE, CC>;
   constexpr int RAD = sched_radix(L, S);
   constexpr int NS = sched_ns(L, S);
@@ -84,7 +85,8 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* 
     static_assert(NB == 1, "split radix-16 stages hold one butterfly per lane");
     __syncthreads();  // every wave finished reading the stage buffer it is about to borrow
     float* scratch = reinterpret_cast<float*>(lds) + (threadIdx.x >> 6) * SPLIT_WAVE_FLOATS;
-    dft16_split_mfma(reinterpret_cast<cpx<float>*>(v), scratch);
+    if constexpr (TAB) dft16_split_mfma_tab(reinterpret_cast<cpx<float>*>(v), scratch, atab);
+    else dft16_split_mfma(reinterpret_cast<cpx<float>*>(v), scratch);
     __syncthreads();  // scratch is released before the stage buffer is written again
   } else {
 #pragma unroll

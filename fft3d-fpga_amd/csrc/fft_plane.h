@@ -81,6 +81,7 @@ struct CubeArgs {
   const float* w_store;
   unsigned long long w_mask;
   int inverse;
+  int split;                     // radix-16 stages on FP16 MFMA with hi + lo operands (no weights)
   // optional transform table (n = 128 only, no weights): block t starts at
   // tab[2 t] in `in` and tab[2 t + 1] in `out` (complex elements); NULL:
   // block t at t n^3 in both

@@ -51,14 +51,11 @@ __device__ __forceinline__ void split2(float x, _Float16* hi, _Float16* lo) {
   *lo = (_Float16)(x - (float)h);
 }
 
-// Radix-16 DFT (forward sign) of the 16 points v[0..15] held by every lane of
-// the wave, in place.  All 64 lanes must call it together; `scratch` is this
-// wave's private LDS region of SPLIT_WAVE_FLOATS floats.
-__device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) {
-  const int lane = threadIdx.x & 63;
+// The constant DFT operands of lane `lane`: A1 = [C | S], A2 = [-S | C],
+// split hi/lo (the A layout of the product above).
+__device__ __forceinline__ void split_a_operands(int lane, split_half8& a1h, split_half8& a1l, split_half8& a2h,
+                                                 split_half8& a2l) {
   const int row = lane & 15, part = lane >> 4;
-  // ---- constant DFT operands A1 = [C | S], A2 = [-S | C], split hi/lo
-  split_half8 a1h, a1l, a2h, a2l;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int t = 8 * (part & 1) + j;  // point index inside the re (part 0,1) or im (2,3) half
@@ -74,6 +71,35 @@ __device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) 
     a2h[j] = h;
     a2l[j] = l;
   }
+}
+
+// A workgroup's table of those operands in LDS, SPLIT_ATAB entries: lane l's
+// four at [4 l .. 4 l + 3].  Filled by the first wave; read after a barrier.
+constexpr int SPLIT_ATAB = 4 * 64;
+__device__ __forceinline__ void split_atab_fill(split_half8* atab) {
+  if (threadIdx.x < 64) {
+    split_half8 a1h, a1l, a2h, a2l;
+    split_a_operands((int)threadIdx.x, a1h, a1l, a2h, a2l);
+    atab[4 * threadIdx.x] = a1h;
+    atab[4 * threadIdx.x + 1] = a1l;
+    atab[4 * threadIdx.x + 2] = a2h;
+    atab[4 * threadIdx.x + 3] = a2l;
+  }
+}
+
+// Radix-16 DFT (forward sign) of the 16 points v[0..15] held by every lane of
+// the wave, in place.  All 64 lanes must call it together; `scratch` is this
+// wave's private LDS region of SPLIT_WAVE_FLOATS floats.  TAB: the constant
+// operands are read from `atab` (an LDS table from split_atab_fill) at every
+// call, and every index of the call is one the compiler cannot see through, so
+// nothing of one call (operands, scratch addresses) stays live until the next;
+// without, they are formed in registers (and a kernel keeps them live across
+// its calls).
+template <bool TAB>
+__device__ __forceinline__ void dft16_split_mfma_impl(cpx<float>* v, float* scratch, const split_half8* atab) {
+  int lane = threadIdx.x & 63;
+  if constexpr (TAB) asm volatile("" : "+v"(lane));
+  const int row = lane & 15, part = lane >> 4;
   // ---- this lane's vector -> scratch row `lane`
   float* my = scratch + lane * SPLIT_ROW;
 #pragma unroll
@@ -84,6 +110,19 @@ __device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // ---- constant operands, formed once this lane's points are out of registers
+  split_half8 a1h, a1l, a2h, a2l;
+  if constexpr (TAB) {
+    int li = lane;
+    asm volatile("" : "+v"(li));
+    a1h = atab[4 * li];
+    a1l = atab[4 * li + 1];
+    a2h = atab[4 * li + 2];
+    a2l = atab[4 * li + 3];
+  } else {
+    split_a_operands(lane, a1h, a1l, a2h, a2l);
+  }
+
   // Group g reads rows 16g..16g+15 and writes its results back into the same
   // rows: the wave issues every read of the group before any of its writes,
   // and one wave's LDS operations complete in order, so each group is done in
@@ -137,6 +176,38 @@ __device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (int t = 0; t < 16; ++t) v[t] = {my[t], my[16 + t]};
+}
+
+__device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) {
+  dft16_split_mfma_impl<false>(v, scratch, nullptr);
+}
+__device__ __forceinline__ void dft16_split_mfma_tab(cpx<float>* v, float* scratch, const split_half8* atab) {
+  dft16_split_mfma_impl<true>(v, scratch, atab);
+}
+
+// Complex elements of a stage buffer of `base` elements that split-precision
+// variants (wg_dft16) also borrow as operand scratch for `nt` threads.
+constexpr int split_lds_cpx(int base, int nt) {
+  return base > (nt / 64) * SPLIT_WAVE_FLOATS / 2 ? base : (nt / 64) * SPLIT_WAVE_FLOATS / 2;
+}
+
+// Radix-16 DFT of v[0..15] in a workgroup kernel whose stage buffer `buf` is
+// free at this point (every thread has read what it needs from it) and holds
+// at least SPLIT_WAVE_FLOATS floats per wave: with SPLIT each wave borrows its
+// slice as operand scratch between two barriers, so every thread of the
+// workgroup must make the call (TAB: the constant operands come from the LDS
+// table `atab`); without, the FP32 butterfly.
+template <bool SPLIT, bool TAB = false>
+__device__ __forceinline__ void wg_dft16(cpx<float>* v, cpx<float>* buf, const split_half8* atab = nullptr) {
+  if constexpr (SPLIT) {
+    __syncthreads();  // every wave finished reading the buffer it is about to borrow
+    float* scratch = reinterpret_cast<float*>(buf) + (threadIdx.x >> 6) * SPLIT_WAVE_FLOATS;
+    if constexpr (TAB) dft16_split_mfma_tab(v, scratch, atab);
+    else dft16_split_mfma(v, scratch);
+    __syncthreads();  // scratch is released before the buffer is written again
+  } else {
+    dft16(v);
+  }
 }
 
 }  // namespace fftamd

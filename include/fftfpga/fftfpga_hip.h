@@ -45,7 +45,8 @@ extern "C" {
 /* Split precision: radix-16 stages as 16x16 complex DFT products on FP16 matrix
  * cores (v_mfma_f32_16x16x32_f16) with hi+lo split operands, power-of-two block
  * scaling and FP32 accumulation; other stages and twiddles stay FP32.  Single
- * precision only; lines of 16..4096 points use it. */
+ * precision only; lines of 16..4096 points use it, and so do the y-split pairs
+ * of 128^3, 256^3 and 512^3 blocks (transforms without weights). */
 #define FFTAMD_SPLIT_MFMA 0x2u
 /* Single-precision transforms whose two fastest axes form planes up to
  * 128 x 128, or 256 x 256 (from 32 planes up), run both axes in one fused pass
@@ -58,8 +59,9 @@ extern "C" {
  * 128 x 128 x 128 run in two passes with the y axis split (n/4) x 4 between
  * them (x and (n/4)-point y over one residue class of rows per workgroup; then
  * the 4-point y stage and z), so that every tile fits one workgroup and no
- * input is read twice.  This flag keeps the plane pass and a separate z pass
- * instead. */
+ * input is read twice; 512 x 512 x 512 likewise with y split 64 x 8.  This
+ * flag keeps the plane pass and a separate z pass instead (three line passes
+ * for 512^3). */
 #define FFTAMD_NO_CUBE_SPLIT 0x8u
 
 /* Single precision, rank 1..3, every dimension a power of two, `batch`
@@ -140,7 +142,7 @@ int fftamd_multi_tune(fftamd_multi_t plan, void *stream, const char *cache_path,
                       double *best_us);
 /* mode 0: one launch with grid barriers; mode 1: one launch per phase;
  * mode 2 (when every transform has three axes of >= 16 points with x-y planes
- * of at most 128 x 128, single precision without split): per shape group, the
+ * of at most 128 x 128): per shape group, the
  * x and y passes as one plane-kernel launch (a whole plane per workgroup, in
  * LDS) and the z passes as one line-pass launch, both over a table of the
  * group's transforms (128^3 groups whose inputs and outputs do not overlap:

@@ -258,9 +258,12 @@ def test_plan_shapes_of_headline_configs(fl):
     # 256 x 256 planes go through the plane pass from 32 planes up
     assert [q["mode"] for q in describe(fl, [256, 256], 32)[0]] == [2]
     assert [q["mode"] for q in describe(fl, [256, 256], 16)[0]] == [0, 1]
-    # double precision and split mode keep the separate passes
+    # double precision keeps the separate passes; split mode runs the y-split
+    # pairs with split radix-16 stages, and without them the three line passes
     assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, prec=1)[0]] == [0, 1, 1]
-    assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, flags=2)[0]] == [0, 1, 1]
+    for n in (128, 256, 512):
+        assert [q["mode"] for q in describe(fl, [n, n, n], 1, flags=2)[0]] == [4, 5]
+    assert [q["mode"] for q in describe(fl, [256, 256, 256], 1, flags=2 | 8)[0]] == [0, 1, 1]
     # 512^3 blocks: the y-split pair with y = a + 8 b, 1024-thread workgroups
     p, fc = describe(fl, [512, 512, 512], 1)
     assert [(q["mode"], q["lg"], q["NT"], q["C"]) for q in p] == [(4, 9, 1024, 64), (5, 9, 1024, 8)] and not fc

@@ -17,12 +17,17 @@ def announce(request):
     sys.__stderr__.flush()
 
 
-@pytest.mark.parametrize("shape,rank", [((16, 16, 16), 3), ((2, 64, 64, 64), 3), ((256, 256, 256), 3),
-                                        ((32, 128, 64), 3), ((4096, 4096), 2), ((1 << 22,), 1),
-                                        ((8, 4096), 1), ((3, 512, 1024), 2), ((4, 128, 128), 2),
-                                        ((2, 16, 128, 128), 3)])
+@pytest.mark.parametrize("shape,rank,cube_split", [((16, 16, 16), 3, True), ((2, 64, 64, 64), 3, True),
+                                                   ((256, 256, 256), 3, True), ((256, 256, 256), 3, False),
+                                                   ((2, 128, 128, 128), 3, True), ((128, 128, 128), 3, False),
+                                                   ((32, 128, 64), 3, True), ((4096, 4096), 2, True),
+                                                   ((1 << 22,), 1, True), ((8, 4096), 1, True),
+                                                   ((3, 512, 1024), 2, True), ((4, 128, 128), 2, True),
+                                                   ((2, 16, 128, 128), 3, True)])
 @pytest.mark.parametrize("inv", [False, True])
-def test_split_vs_oracle(gpu_lib, orc, shape, rank, inv):
+def test_split_vs_oracle(gpu_lib, orc, shape, rank, cube_split, inv):
+    """128^3 and 256^3 blocks run the y-split pair with split radix-16 stages
+    (cube_split=False: the plane pass and z pass, as other 3D shapes)."""
     import torch
     from fftfpga_amd import c2c_dev
     n = int(np.prod(shape))
@@ -30,13 +35,13 @@ def test_split_vs_oracle(gpu_lib, orc, shape, rank, inv):
     dims = list(shape[len(shape) - rank:])
     batch = n // int(np.prod(dims))
     xt = torch.from_numpy(x).cuda()
-    y = c2c_dev(xt, rank, inverse=inv, split=True)
+    y = c2c_dev(xt, rank, inverse=inv, split=True, cube_split=cube_split)
     yp = c2c_dev(xt, rank, inverse=inv)
     torch.cuda.synchronize()
     ref = orc.fftnd(x, dims, batch=batch, inverse=inv)
     err = orc.rel_l2(y.cpu().numpy(), ref)
     errp = orc.rel_l2(yp.cpu().numpy(), ref)
-    print(f"split {shape} rank={rank} inv={inv} rel_l2={err:.3e} (plain fp32 {errp:.3e})")
+    print(f"split {shape} rank={rank} cube_split={cube_split} inv={inv} rel_l2={err:.3e} (plain fp32 {errp:.3e})")
     assert err <= TOL
 
 
