@@ -9,9 +9,10 @@
 // FP16 carries 11 significant bits, so every operand is split as
 // x = hi + lo (hi = fp16(x), lo = fp16(x - hi)) and the product is
 // A_hi B_hi + A_hi B_lo + A_lo B_hi (the dropped A_lo B_lo term is ~2^-23
-// relative): 6 MFMAs per 16 vectors.  Before splitting, each group of 16
-// vectors is scaled by a power of two so its largest component lies in
-// [2^14, 2^15): FP16 cannot overflow, and the scale is exact.
+// relative): 6 MFMAs per 16 vectors.  Before splitting, each vector (a column
+// of B, and of the product) is scaled by a power of two so its largest
+// component lies in [2^14, 2^15): FP16 cannot overflow, the scale is exact,
+// and a vector's precision does not depend on the other vectors' magnitudes.
 //
 // Operand layout of v_mfma_f32_16x16x32_f16 (checked on the device with an
 // integer-valued probe): lane l holds A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16]
@@ -138,9 +139,11 @@ __device__ __forceinline__ void dft16_split_mfma_impl(cpx<float>* v, float* scra
       x[j] = rowp[8 * part + j];
       mx = fmaxf(mx, fabsf(x[j]));
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    // power-of-two scale bringing the group maximum into [2^14, 2^15)
+    // the vector's maximum over its four lanes (parts): each output column
+    // depends on its own vector only, so every vector takes its own scale
+    mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    // power-of-two scale bringing the vector's maximum into [2^14, 2^15)
     int ex = 0;
     (void)frexpf(mx, &ex);  // mx = f * 2^ex, f in [0.5, 1)
     const int sh = mx > 0.0f ? 15 - ex : 0;

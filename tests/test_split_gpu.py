@@ -74,7 +74,7 @@ def test_split_dynamic_range_512cubed_with_spikes(gpu_lib, orc):
 
 def test_split_constant_input_does_not_overflow(gpu_lib):
     """A constant 256^3 input drives the DC term to 2^24, far past FP16's range;
-    per-group power-of-two scaling keeps every operand representable."""
+    the per-vector power-of-two scaling keeps every operand representable."""
     import torch
     from fftfpga_amd import c2c_dev
     N = 256
@@ -86,3 +86,20 @@ def test_split_constant_input_does_not_overflow(gpu_lib):
     assert abs(dc - (1.0 + 0.5j) * N ** 3) <= 1e-6 * abs(dc)
     y[0, 0, 0] = 0
     assert torch.abs(y).max().item() <= 1e-6 * abs(dc)
+
+
+def test_split_lines_of_very_different_magnitudes(gpu_lib, orc):
+    """64-point lines put four lines in each group of 16 vectors of a split
+    stage; with lines alternately 1e8 and 1 in magnitude, every line must keep
+    the bar on its own (each vector is scaled by its own maximum, so the small
+    lines' low parts do not fall into FP16 subnormals)."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    L, B = 64, 64
+    x = orc.uniform(L * B, 77).reshape(B, L)
+    x[0::2] *= 1e8
+    y = c2c_dev(torch.from_numpy(x).cuda(), 1, split=True).cpu().numpy()
+    ref = orc.fftnd(x.reshape(-1), [L], batch=B).reshape(B, L)
+    errs = [orc.rel_l2(y[i], ref[i]) for i in range(B)]
+    print(f"per-line rel_l2: large lines max {max(errs[0::2]):.3e}, small lines max {max(errs[1::2]):.3e}")
+    assert max(errs) <= TOL
