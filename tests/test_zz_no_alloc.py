@@ -22,12 +22,12 @@ def announce(request):
     sys.__stderr__.flush()
 
 
-def _call(lib, fn, dims, batch, x, out, inverse):
+def _call(lib, fn, dims, batch, x, out, inverse, flags=0):
     import torch
     d = (ctypes.c_uint * len(dims))(*dims)
     s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     rc = getattr(lib, fn)(len(dims), d, batch, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                          1 if inverse else 0, 0, s)
+                          1 if inverse else 0, flags, s)
     return rc
 
 
@@ -66,6 +66,24 @@ def test_noalloc_matches_allocating_entry(fl, orc, first, second, inverse):
               f"identical={np.array_equal(got, want)}")
         assert err <= 2e-6
         assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_noalloc_split_matches_allocating_entry(fl, orc, inverse):
+    """Split precision (FFTAMD_SPLIT_MFMA) through the allocation-free entry
+    point: the y-split pairs' split stages keep their operand table in LDS, so
+    nothing is allocated; 256^3 then 2 x 128^3, identical to fftamd_c2c_dev."""
+    import torch
+    lib = fl.load()
+    for k, (dims, batch) in enumerate(([[256, 256, 256], 1], [[128, 128, 128], 2])):
+        n = int(np.prod(dims)) * batch
+        x = torch.from_numpy(orc.uniform(n, 90 + k + (5 if inverse else 0))).cuda()
+        ref = torch.empty_like(x)
+        y = torch.empty_like(x)
+        assert _call(lib, "fftamd_c2c_dev", dims, batch, x, ref, inverse, flags=2) == 0, fl.last_error()
+        assert _call(lib, "fftamd_c2c_dev_noalloc", dims, batch, x, y, inverse, flags=2) == 0, fl.last_error()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref), f"call {k}: {dims} x {batch}"
 
 
 def test_noalloc_rejects_scratch_overflow(fl):
