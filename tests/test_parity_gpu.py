@@ -675,3 +675,23 @@ def test_cube512_weights(gpu_lib, orc):
         err = orc.rel_l2(y.cpu().numpy(), ref)
         print(f"512^3 weighted inv={inv} rel_l2={err:.3e}")
         assert err <= TOL
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_cube512_batch_of_two_blocks(gpu_lib, orc, split):
+    """Two 512^3 blocks in one call (pass B decodes the block from its
+    blockIdx): each equals that block transformed on its own, bit for bit, in
+    both directions, in FP32 and with split-precision stages."""
+    import torch
+    from fftfpga_amd import c2c_dev
+    N = 512
+    x = torch.from_numpy(orc.uniform(2 * N ** 3, 5124 + int(split))).cuda().view(2, N, N, N)
+    for inv in (False, True):
+        y = c2c_dev(x, 3, inverse=inv, split=split)
+        for b in range(2):
+            yb = c2c_dev(x[b].contiguous(), 3, inverse=inv, split=split)
+            torch.cuda.synchronize()
+            assert torch.equal(y[b], yb), f"block {b} inv={inv} split={split}"
+        del y, yb
+    del x
+    torch.cuda.empty_cache()
