@@ -67,6 +67,9 @@ def flops_per_transform(dims):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--cpu-fftw-child", action="store_true",
+                    help="only time the FFTW-API CPU baseline of the workload and print it (run by the "
+                         "bench itself in a child process that never loads torch)")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="3d_256_fwd_inv", choices=sorted(WORKLOADS))
@@ -472,12 +475,34 @@ def batched_vs_plain(steps, warmup):
     }
 
 
+def cpu_fftwf_isolated(workload, dims, seed):
+    """cpu_fftwf in a child process of this script that never imports torch:
+    the FFTW API's threads (MKL's OpenMP runtime, when the API comes from MKL)
+    then do not share the process with torch's own OpenMP pool.  Falls back to
+    measuring in this process if the child fails."""
+    import subprocess
+    try:
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--workload", workload, "--cpu-fftw-child"],
+                           capture_output=True, text=True, timeout=600)
+        r = json.loads(p.stdout.strip().splitlines()[-1])
+        r["process"] = "child process without torch"
+        return r
+    except Exception as e:  # noqa: BLE001 -- any failure of the child falls back
+        r = cpu_fftwf(dims, seed)
+        r["process"] = f"bench process (child failed: {type(e).__name__})"
+        return r
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not a bench line)"}))
         sys.exit(2)
+    if args.cpu_fftw_child:
+        dims, _, seed, _ = WORKLOADS[args.workload]
+        print(json.dumps(cpu_fftwf(dims, seed)), flush=True)
+        return
     import torch
 
     import fftfpga_amd as fl
@@ -659,7 +684,7 @@ def main():
         accuracy["rel_l2_forward_vs_fp64_oracle"] = err
         if not args.no_cpu_fftw:
             # the reference's own CPU path (FFTW single precision) on this host
-            result["cpu_baseline"] = cpu_fftwf(dims, seed)
+            result["cpu_baseline"] = cpu_fftwf_isolated(args.workload, dims, seed)
     result["accuracy"] = accuracy
     result["cpu_reference_emulation"] = {
         "available": False,
