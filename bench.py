@@ -438,6 +438,27 @@ def batched_vs_plain(steps, warmup):
     ms_eager = run(plan)
     ms_plain = run(plain)
     plan.check()
+    # the whole batch as one persistent dataflow launch (mode 3), whatever the tuner chose
+    single = None
+    if tune["mode"] != 3:
+        sch = plan.schedule()
+        best = None
+        for per_cu, chunk in ((2, 4), (2, 8)):
+            ncu = torch.cuda.get_device_properties(0).multi_processor_count
+            try:
+                plan.set_schedule(3, per_cu * ncu, chunk)
+            except ValueError:
+                break
+            ms = run(plan)
+            plan.check()
+            if best is None or ms < best[0]:
+                best = (ms, per_cu, chunk)
+        plan.set_schedule(sch["mode"], sch["grid"], sch["chunk"])
+        if best:
+            single = {"ms_eager": round(best[0], 4), "workgroups_per_cu": best[1], "chunk": best[2],
+                      "over_tuned_graph": round(best[0] / ms_graph, 3)}
+    else:
+        single = {"ms_eager": round(ms_eager, 4), "over_tuned_graph": round(ms_eager / ms_graph, 3)}
     # the same batch with split-precision (FP16 matrix-core) radix-16 stages
     d_split = torch.empty_like(d_in)
     splan = MultiPlan(shapes, d_in, d_split, split=True)
@@ -449,10 +470,10 @@ def batched_vs_plain(steps, warmup):
     flops = sum(flops_per_transform(sh) for sh in shapes)
     alg = 16.0 * total
     # the per-transform path with the same kernels as the tuned schedule
-    # (mode 2: plane kernels, and the y-split pair for the 128^3 members;
+    # (modes 2 and 3: plane kernels, and the y-split pair for the 128^3 members;
     # modes 0, 1: line passes) must give the same bits
     for a, b in views:
-        fl.c2c_dev(a, 3, out=b, plane_fusion=(tune["mode"] == 2))
+        fl.c2c_dev(a, 3, out=b, plane_fusion=(tune["mode"] in (2, 3)))
     torch.cuda.synchronize()
     same = bool(torch.equal(d_out, d_ref))
     return {
@@ -468,6 +489,7 @@ def batched_vs_plain(steps, warmup):
         "roofline_frac_batched": round(alg / (ms_graph * 1e-3) / HBM_PEAK, 4),
         "schedule": {k: tune[k] for k in ("mode", "grid", "chunk", "items", "phases", "from_cache")},
         "outputs_identical_to_per_transform_path": same,
+        "single_persistent_launch": single,
         "split_precision": {"ms_batched_graph": round(ms_split, 4),
                             "speedup_vs_plain": round(ms_plain / ms_split, 2),
                             "rel_l2_vs_fp32_path": split_rel,

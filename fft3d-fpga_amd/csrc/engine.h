@@ -78,6 +78,10 @@ void graph_destroy(void* graph);
 int describe_plan(int rank, const size_t* dims, size_t batch, unsigned flags, int prec, bool aliased,
                   long long* out, int cap);
 
+// Host-only description of the dataflow schedule's queues (see engine.hip);
+// returns the tile count or -1.
+int flow_describe(unsigned count, const unsigned* dims3, unsigned* qstart, unsigned* out, unsigned cap);
+
 // Pass profiling: while on, every pass launch is bracketed by HIP events on
 // its stream; profile_end synchronises and writes a JSON summary
 // {label: {count, total_ms}}.

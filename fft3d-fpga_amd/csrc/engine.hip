@@ -966,6 +966,16 @@ struct MultiPlan {
   std::vector<hipStream_t> aux;
   std::vector<hipEvent_t> aux_done;
   hipEvent_t fork_ev = nullptr;
+  // mode 3 (one dataflow launch, fft_flow.hip): every transform n^3 with
+  // n = 32, 64 or 128 (128^3 out of place), FP32
+  bool flow_ok = false;
+  FlowArgs flow{};
+  std::vector<FlowItem> fitems;
+  std::vector<unsigned> fq;        // FLOW_QUEUES + 1 queue starts
+  FlowItem* d_fitems = nullptr;
+  unsigned* d_fq = nullptr;
+  unsigned* d_done = nullptr;      // finished first-pass tiles per transform
+  unsigned ndone = 0;
   ~MultiPlan() {
     for (hipStream_t s : aux) (void)hipStreamDestroy(s);
     for (hipEvent_t e : aux_done) (void)hipEventDestroy(e);
@@ -976,15 +986,20 @@ struct MultiPlan {
     if (d_ctl) (void)hipFree(d_ctl);
     if (d_tab) (void)hipFree(d_tab);
     if (d_tab_z) (void)hipFree(d_tab_z);
+    if (d_fitems) (void)hipFree(d_fitems);
+    if (d_fq) (void)hipFree(d_fq);
+    if (d_done) (void)hipFree(d_done);
   }
 };
 
 static unsigned g_multi_spin_limit = 1u << 22;
 
-static int multi_default_grid(unsigned* grid, unsigned* per_cu_max, bool split) {
+// Resident workgroups of the persistent kernel of `mode` (3: the dataflow
+// kernel; else the line-tile kernel) per CU and over the device.
+static int multi_default_grid(unsigned* grid, unsigned* per_cu_max, bool split, int mode = 0) {
   int ncu = 0, per = 0;
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, S().dev < 0 ? 0 : S().dev));
-  if (multi_max_blocks_per_cu(&per, split) || per <= 0) {
+  if ((mode == 3 ? flow_max_blocks_per_cu(&per) : multi_max_blocks_per_cu(&per, split)) || per <= 0) {
     set_error("occupancy query for the multi-transform kernel failed");
     return -1;
   }
@@ -994,6 +1009,144 @@ static int multi_default_grid(unsigned* grid, unsigned* per_cu_max, bool split) 
 }
 
 static int multi_prepare_tables(const MultiPlan* mp);
+static int group_cube_args(const MultiPlan* mp, const PlaneGroup& g, CubeArgs* ca, CubeArgs* cb);
+static int group_plane_args(const MultiPlan* mp, const PlaneGroup& g, PlaneArgs* pa, PassArgs* za);
+
+// ---- dataflow launch (mode 3) ----------------------------------------------------
+// A transform of the launch: its kind (0: 32^3, 1: 64^3, 2: 128^3) and its
+// index within its group.
+struct FlowXf {
+  unsigned char kind;
+  unsigned tr;
+};
+static const int kFlowSide[3] = {32, 64, 128};
+// Queues of the dataflow launch.  Transforms go to the queues largest first,
+// each to the queue with the fewest bytes so far (lowest index on ties); a
+// queue lists the first-pass tiles of its transforms in that order, then
+// their second-pass tiles in the same order.  Every second-pass tile thus
+// comes after all first-pass tiles of its transform, in the same queue.
+// (Second-pass tiles interleaved among the first-pass ones, each transform's
+// 256 or 2048 tiles after its last first-pass tile, measured 1.53 and 0.96 ms
+// on the 448-transform batch against 0.94 ms: a workgroup publishes before
+// it waits, so every switch between the passes costs a release.)
+static void flow_layout(const std::vector<FlowXf>& xf, std::vector<FlowItem>* items,
+                        std::vector<unsigned>* qstart) {
+  std::vector<unsigned> ord(xf.size());
+  std::iota(ord.begin(), ord.end(), 0u);
+  std::stable_sort(ord.begin(), ord.end(), [&](unsigned a, unsigned b) { return xf[a].kind > xf[b].kind; });
+  std::vector<std::vector<unsigned>> q(FLOW_QUEUES);
+  std::vector<unsigned long long> load(FLOW_QUEUES, 0);
+  for (unsigned i : ord) {
+    const unsigned m = (unsigned)(std::min_element(load.begin(), load.end()) - load.begin());
+    const unsigned long long n = (unsigned long long)kFlowSide[xf[i].kind];
+    q[m].push_back(i);
+    load[m] += n * n * n;
+  }
+  items->clear();
+  qstart->assign(1, 0u);
+  for (const std::vector<unsigned>& qq : q) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (unsigned i : qq) {
+        const unsigned nt = flow_tiles(kFlowSide[xf[i].kind], pass);
+        for (unsigned sub = 0; sub < nt; ++sub)
+          items->push_back({xf[i].tr, (unsigned short)sub, xf[i].kind, (unsigned char)pass});
+      }
+    qstart->push_back((unsigned)items->size());
+  }
+}
+
+// The group's kind for the dataflow launch, or -1 when it has none.
+static int flow_kind(const PlaneGroup& g) {
+  if (g.ly != (int)g.ppt || g.lx != (int)g.ppt) return -1;
+  if (g.ppt == 32) return 0;
+  if (g.ppt == 64) return 1;
+  if (g.ppt == 128 && g.cube) return 2;
+  return -1;
+}
+
+// The dataflow launch's groups, queues and counters (FP32 plans whose groups
+// all have a kind); leaves flow_ok false otherwise.
+static int flow_build(MultiPlan* mp) {
+  if (mp->split) return 0;
+  std::vector<FlowXf> xf;
+  unsigned dep = 0;
+  for (const PlaneGroup& g : mp->groups)
+    if (flow_kind(g) < 0) return 0;
+  for (const PlaneGroup& g : mp->groups) {
+    const int k = flow_kind(g);
+    FlowGroup& fg = mp->flow.grp[k];
+    fg.n = (int)g.ppt;
+    fg.dep0 = dep;
+    fg.need = flow_tiles(fg.n, 0);
+    if (k == 2) {
+      if (group_cube_args(mp, g, &fg.ca, &fg.cb)) return -1;
+    } else {
+      if (group_plane_args(mp, g, &fg.pa, &fg.za)) return -1;
+      if (fg.za.ncb != 2 * flow_tiles(fg.n, 1)) {
+        set_error("dataflow layout: unexpected z-line blocks per transform");
+        return -1;
+      }
+    }
+    for (unsigned tr = 0; tr < g.count; ++tr) xf.push_back({(unsigned char)k, tr});
+    dep += g.count;
+  }
+  flow_layout(xf, &mp->fitems, &mp->fq);
+  mp->ndone = dep;
+  HIP_TRY(hipMalloc(&mp->d_fitems, sizeof(FlowItem) * std::max<size_t>(1, mp->fitems.size())));
+  HIP_TRY(hipMalloc(&mp->d_fq, sizeof(unsigned) * mp->fq.size()));
+  HIP_TRY(hipMalloc(&mp->d_done, sizeof(unsigned) * FLOW_DONE_STRIDE * std::max(1u, dep)));
+  if (!mp->fitems.empty())
+    HIP_TRY(hipMemcpy(mp->d_fitems, mp->fitems.data(), sizeof(FlowItem) * mp->fitems.size(),
+                      hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(mp->d_fq, mp->fq.data(), sizeof(unsigned) * mp->fq.size(), hipMemcpyHostToDevice));
+  mp->flow_ok = true;
+  return 0;
+}
+
+// Host-only: the dataflow queues of a batch of cubes (n = 32, 64, 128), with
+// each tile as (transform index, pass, tile) -- grouped by side, largest
+// first, members in index order, as multi_create groups them.
+int flow_describe(unsigned count, const unsigned* dims3, unsigned* qstart, unsigned* out, unsigned cap) {
+  if (!dims3 || !qstart || count == 0) {
+    set_error("flow_describe: bad arguments");
+    return -1;
+  }
+  std::vector<unsigned> members[3];
+  for (unsigned i = 0; i < count; ++i) {
+    const unsigned n = dims3[3 * i];
+    if (dims3[3 * i + 1] != n || dims3[3 * i + 2] != n || (n != 32 && n != 64 && n != 128)) {
+      set_error("flow_describe: transform %u is not 32^3, 64^3 or 128^3", i);
+      return -1;
+    }
+    members[n == 32 ? 0 : n == 64 ? 1 : 2].push_back(i);
+  }
+  std::vector<FlowXf> xf;
+  std::vector<unsigned> orig;
+  for (int k = 2; k >= 0; --k)
+    for (unsigned tr = 0; tr < members[k].size(); ++tr) {
+      xf.push_back({(unsigned char)k, tr});
+      orig.push_back(members[k][tr]);
+    }
+  std::vector<FlowItem> items;
+  std::vector<unsigned> qs;
+  flow_layout(xf, &items, &qs);
+  for (size_t i = 0; i < qs.size(); ++i) qstart[i] = qs[i];
+  if (!out) return (int)items.size();
+  if (items.size() > cap) {
+    set_error("flow_describe: buffer too small");
+    return -1;
+  }
+  // group-local index back to the caller's index
+  std::vector<unsigned> first(3, 0);
+  for (size_t j = 0; j < xf.size(); ++j)
+    if (xf[j].tr == 0) first[xf[j].kind] = (unsigned)j;
+  for (size_t i = 0; i < items.size(); ++i) {
+    out[3 * i] = orig[first[items[i].group] + items[i].tr];
+    out[3 * i + 1] = items[i].pass;
+    out[3 * i + 2] = items[i].sub;
+  }
+  return (int)items.size();
+}
 
 int multi_create(void** handle, unsigned count, const unsigned* dims3, const unsigned long long* in_off,
                  const unsigned long long* out_off, const void* din, void* dout, bool inverse,
@@ -1162,6 +1315,7 @@ int multi_create(void** handle, unsigned count, const unsigned* dims3, const uns
                       hipMemcpyHostToDevice));
     // made now, so that a mode-2 launch allocates nothing (it may run under capture)
     if (multi_prepare_tables(mp.get())) return -1;
+    if (flow_build(mp.get())) return -1;
   }
   unsigned per = 0;
   if (multi_default_grid(&mp->grid, &per, mp->split)) return -1;
@@ -1205,32 +1359,29 @@ static int multi_prepare_fork(MultiPlan* mp) {
   return 0;
 }
 
-// x and y of one shape group as one plane-kernel launch, then its z axis as
-// one strided line pass over all its transforms, in place
-static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stream) {
-  if (g.cube) {
-    CubeArgs ca{};
-    ca.n = 128;
-    ca.nplanes = (unsigned long long)g.count * 128;
-    if (get_stockham(PREC_F32, 128, &ca.tw)) return -1;
-    if (get_direct(256, &ca.w256)) return -1;
-    ca.inverse = mp->inverse ? 1 : 0;
-    ca.split = mp->split ? 1 : 0;
-    ca.in = mp->din;
-    ca.out = mp->dout;
-    ca.tab = mp->d_tab + 2 * g.tab_off;
-    if (launch_fft_cube_a(ca, stream)) {
-      set_error("y-split pass A launch failed: %s", hipGetErrorString(hipGetLastError()));
-      return -1;
-    }
-    ca.in = mp->dout;  // pass B in place on the output
-    ca.tab = mp->d_tab_z + 2 * g.tab_off;
-    if (launch_fft_cube_b(ca, stream)) {
-      set_error("y-split pass B launch failed: %s", hipGetErrorString(hipGetLastError()));
-      return -1;
-    }
-    return 0;
-  }
+// The y-split pair's arguments for a 128^3 group: pass A out of place from
+// the inputs, pass B in place on the outputs (transform tables).
+static int group_cube_args(const MultiPlan* mp, const PlaneGroup& g, CubeArgs* ca, CubeArgs* cb) {
+  CubeArgs c{};
+  c.n = 128;
+  c.nplanes = (unsigned long long)g.count * 128;
+  if (get_stockham(PREC_F32, 128, &c.tw)) return -1;
+  if (get_direct(256, &c.w256)) return -1;
+  c.inverse = mp->inverse ? 1 : 0;
+  c.split = mp->split ? 1 : 0;
+  c.in = mp->din;
+  c.out = mp->dout;
+  c.tab = mp->d_tab + 2 * g.tab_off;
+  *ca = c;
+  c.in = mp->dout;  // pass B in place on the output
+  c.tab = mp->d_tab_z + 2 * g.tab_off;
+  *cb = c;
+  return 0;
+}
+
+// A plane group's arguments: x and y of every plane from the inputs to the
+// outputs, then the z lines in place on the outputs.
+static int group_plane_args(const MultiPlan* mp, const PlaneGroup& g, PlaneArgs* pa, PassArgs* za) {
   PlaneArgs a{};
   a.in = mp->din;
   a.out = mp->dout;
@@ -1241,10 +1392,6 @@ static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stre
   a.inverse = mp->inverse ? 1 : 0;
   a.tab = mp->d_tab + 2 * g.tab_off;
   a.ppt = g.ppt;
-  if (launch_fft_plane(g.ly, g.lx, a, stream, mp->split)) {
-    set_error("plane kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
-    return -1;
-  }
   const long long L = g.ppt, s = (long long)g.ly * g.lx;
   PassArgs z{};
   z.in = mp->dout;
@@ -1258,6 +1405,35 @@ static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stre
   z.inverse = mp->inverse ? 1 : 0;
   z.tab = mp->d_tab_z + 2 * g.tab_off;
   if (sched_np((int)L) > 1 && get_stockham(PREC_F32, (int)L, &z.tw)) return -1;
+  *pa = a;
+  *za = z;
+  return 0;
+}
+
+// x and y of one shape group as one plane-kernel launch, then its z axis as
+// one strided line pass over all its transforms, in place
+static int multi_group_exec(MultiPlan* mp, const PlaneGroup& g, hipStream_t stream) {
+  if (g.cube) {
+    CubeArgs ca{}, cb{};
+    if (group_cube_args(mp, g, &ca, &cb)) return -1;
+    if (launch_fft_cube_a(ca, stream)) {
+      set_error("y-split pass A launch failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+    if (launch_fft_cube_b(cb, stream)) {
+      set_error("y-split pass B launch failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+    return 0;
+  }
+  PlaneArgs a{};
+  PassArgs z{};
+  if (group_plane_args(mp, g, &a, &z)) return -1;
+  if (launch_fft_plane(g.ly, g.lx, a, stream, mp->split)) {
+    set_error("plane kernel launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return -1;
+  }
+  const long long L = g.ppt;
   // split precision applies to lines with a radix-16 stage, as in the line passes
   if (launch_fft_pass(PREC_F32, ilog2c(L), MODE_STRIDED, EPI_NONE, z, (unsigned long long)z.ncb * g.count,
                       stream, mp->split && L >= 16)) {
@@ -1311,6 +1487,14 @@ int multi_exec(void* handle, void* stream) {
     for (size_t i = 0; i < mp->groups.size(); ++i) HIP_TRY(hipStreamWaitEvent(hs, mp->aux_done[i], 0));
     return 0;
   }
+  if (mp->mode == 3) {
+    if (launch_flow(mp->flow, mp->d_fitems, mp->d_fq, mp->d_ctl, mp->d_done, mp->ndone, mp->grid, mp->chunk,
+                    g_multi_spin_limit, mp->inverse, stream)) {
+      set_error("dataflow launch failed: %s", hipGetErrorString(hipGetLastError()));
+      return -1;
+    }
+    return 0;
+  }
   if (mp->mode == 0) {
     if (launch_multi(mp->d_passes, mp->d_items, mp->d_phase, 0, mp->nphase, mp->d_ctl, mp->grid, mp->chunk,
                      g_multi_spin_limit, stream, mp->split)) {
@@ -1361,10 +1545,12 @@ int multi_get_schedule(void* handle, int* mode, unsigned* grid, unsigned* chunk,
 int multi_set_schedule(void* handle, int mode, unsigned grid, unsigned chunk) {
   MultiPlan* mp = static_cast<MultiPlan*>(handle);
   unsigned g = 0, per = 0;
-  if (!mp || multi_default_grid(&g, &per, mp->split)) return -1;
-  if (mode < 0 || mode > 2 || (mode == 2 && !mp->plane_ok) || grid == 0 || grid > g) {
-    set_error("schedule out of range (modes 0, 1%s; grid must be 1..%u resident workgroups)",
-              mp->plane_ok ? ", 2" : "", g);
+  if (!mp || multi_default_grid(&g, &per, mp->split, mode)) return -1;
+  if (grid == 0) grid = g;  // every workgroup that can be resident
+  if (mode < 0 || mode > 3 || (mode == 2 && !mp->plane_ok) || (mode == 3 && !mp->flow_ok) || grid > g ||
+      (mode == 3 && chunk == 0)) {
+    set_error("schedule out of range (modes 0, 1%s%s; grid must be 0 (all) or 1..%u resident workgroups%s)",
+              mp->plane_ok ? ", 2" : "", mp->flow_ok ? ", 3" : "", g, mode == 3 ? ", chunk >= 1" : "");
     return -1;
   }
   mp->mode = mode;
@@ -1384,9 +1570,12 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
   State& st = S();
   std::lock_guard<std::recursive_mutex> lk(st.mu);
   if (!multi_current(mp)) return -1;
-  unsigned gmax = 0, per = 0;
+  unsigned gmax = 0, per = 0, gflow = 0, per_flow = 0;
   if (multi_default_grid(&gmax, &per, mp->split)) return -1;
+  if (mp->flow_ok && multi_default_grid(&gflow, &per_flow, mp->split, 3)) return -1;
   const unsigned ncu = gmax / per;
+  const int max_mode = mp->flow_ok ? 3 : mp->plane_ok ? 2 : 1;
+  auto per_of = [&](int mode) { return mode == 3 ? per_flow : per; };
   // cached?
   if (cache_path && *cache_path) {
     std::ifstream f(cache_path);
@@ -1397,8 +1586,8 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
       int mode;
       unsigned pc, ch;
       double us;
-      if ((ls >> sg >> mode >> pc >> ch >> us) && sg == mp->sig && pc >= 1 && pc <= per && mode >= 0 &&
-          mode <= (mp->plane_ok ? 2 : 1)) {
+      if ((ls >> sg >> mode >> pc >> ch >> us) && sg == mp->sig && mode >= 0 && mode <= max_mode &&
+          (mode != 2 || mp->plane_ok) && pc >= 1 && pc <= per_of(mode) && (mode != 3 || ch >= 1)) {
         mp->mode = mode;
         mp->grid = pc * ncu;
         mp->chunk = ch;
@@ -1431,11 +1620,15 @@ int multi_tune(void* handle, void* stream, const char* cache_path, int reps, dou
   int bmode = 0;
   unsigned bpc = per, bch = 4;
   const unsigned chunks[] = {1, 2, 4, 8, 16, 0};  // 0: static round-robin, no tickets
-  for (int mode = 0; mode <= (mp->plane_ok ? 2 : 1); ++mode)
-    for (unsigned pc = 1; pc <= per; ++pc)
+  for (int mode = 0; mode <= max_mode; ++mode)
+    for (unsigned pc = 1; pc <= per_of(mode); ++pc)
       for (unsigned ch : chunks) {
         // mode 2 launches plain kernels: grid and chunk do not apply
         if (mode == 2 && (pc != per || ch != chunks[0])) continue;
+        if (mode == 2 && !mp->plane_ok) continue;
+        // mode 3 takes its tiles by tickets only (its queues' order is what
+        // lets a launch finish with any number of resident workgroups)
+        if (mode == 3 && ch == 0) continue;
         mp->mode = mode;
         mp->grid = pc * ncu;
         mp->chunk = ch;
@@ -1602,6 +1795,7 @@ int runtime_init(int dev, bool svm) {
   preload_col();
   preload_plane();
   preload_multi();
+  preload_flow();
   st.dev = dev;
   st.svm = svm;
   st.ready = true;

@@ -144,4 +144,9 @@ int fftamd_describe_plan(int rank, const unsigned* dims, unsigned batch, unsigne
   return fftamd::describe_plan(rank, d, batch, flags, prec, aliased != 0, out, cap);
 }
 
+int fftamd_describe_flow(unsigned count, const unsigned* dims3, unsigned* qstart, unsigned* out,
+                         unsigned cap) {
+  return fftamd::flow_describe(count, dims3, qstart, out, cap);
+}
+
 }  // extern "C"

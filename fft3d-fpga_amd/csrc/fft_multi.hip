@@ -25,7 +25,7 @@ template <int MODE, int LG, bool SPLIT>
 __device__ __forceinline__ void run_path_body(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
   static_assert(geom_threads(1 << LG, MODE) == MULTI_THREADS, "multi paths use 256-thread tiles");
   // split precision applies to lines with a radix-16 stage
-  pass_block<float, (1 << LG), MODE, EPI_NONE, SPLIT && (LG >= 4)>(a, blk, lds);
+  pass_block<float, (1 << LG), MODE, EPI_NONE, SPLIT && (LG >= 4)>(a, blk, lds, (int)threadIdx.x);
 }
 
 // Split-precision paths run as one call per tile: out of line, each gets its

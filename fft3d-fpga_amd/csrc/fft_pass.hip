@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(geom_threads_for(L, MODE, SPLIT))
   using G = Geom<L, MODE, CC>;
   constexpr bool NEED_LDS = G::NP > 1 || SPLIT;
   __shared__ cpx<R> lds[NEED_LDS ? (SPLIT ? G::LDS_ELEMS_SPLIT : G::LDS_ELEMS) : 1];
-  pass_block<R, L, MODE, EPI, SPLIT, DIR, CC>(a, blockIdx.x, lds);
+  pass_block<R, L, MODE, EPI, SPLIT, DIR, CC>(a, blockIdx.x, lds, (int)threadIdx.x);
 }
 
 // ---- launch table ------------------------------------------------------------

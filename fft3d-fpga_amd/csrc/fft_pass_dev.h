@@ -153,10 +153,9 @@ struct TileLoc {
 };
 
 template <int L, int MODE, int CC = geom_cols(L, MODE)>
-__device__ __forceinline__ TileLoc tile_locate(const PassArgs& a, unsigned long long g) {
+__device__ __forceinline__ TileLoc tile_locate(const PassArgs& a, unsigned long long g, int tid) {
   using G = Geom<L, MODE, CC>;
   TileLoc t;
-  const int tid = threadIdx.x;
   t.c = MODE == MODE_CONTIG ? tid / G::TPL : tid % G::C;
   t.tau = MODE == MODE_CONTIG ? tid % G::TPL : tid / G::C;
   const unsigned cb = (unsigned)(g % a.ncb);
@@ -320,13 +319,18 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
   }
 }
 
-// One workgroup tile of a pass (block index g).  Used by k_fft_pass (one tile
-// per workgroup) and by the persistent multi-transform kernel.
+// One workgroup tile of a pass (block index g), run by threads tid = 0 ..
+// Geom::NT - 1.  Used by k_fft_pass (one tile per workgroup) and by the
+// persistent multi-transform kernels (which may run two tiles side by side in
+// one workgroup, each half with its own tid range and LDS; split-precision
+// stages index their per-wave scratch by threadIdx.x and so need the whole
+// workgroup to be one tile).
 template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1,
           int CC = geom_cols(L, MODE)>
-__device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds) {
+__device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds,
+                                           int tid) {
   cpx<R> v[Geom<L, MODE, CC>::PPT];
-  const TileLoc t = tile_locate<L, MODE, CC>(a, g);
+  const TileLoc t = tile_locate<L, MODE, CC>(a, g, tid);
   tile_load<R, L, MODE, CC>(a, t, v);
   tile_finish<R, L, MODE, EPI, SPLIT, DIR, CC>(a, t, v, lds);
 }

@@ -24,6 +24,7 @@ void preload_xy();
 void preload_col();
 void preload_plane();
 void preload_multi();
+void preload_flow();
 inline void preload_kernel(const void* f) {
   hipFuncAttributes fa;
   (void)hipFuncGetAttributes(&fa, f);

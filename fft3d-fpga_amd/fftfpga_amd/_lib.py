@@ -91,6 +91,7 @@ SIGNATURES = {
     "fftamd_profile_end": (_i, [ctypes.c_char_p, _i]),
     "fftamd_describe_plan": (_i, [_i, ctypes.POINTER(_u), _u, _u, _i, _i,
                                   ctypes.POINTER(ctypes.c_longlong), _i]),
+    "fftamd_describe_flow": (_i, [_u, ctypes.POINTER(_u), ctypes.POINTER(_u), ctypes.POINTER(_u), _u]),
 }
 
 _lib = None

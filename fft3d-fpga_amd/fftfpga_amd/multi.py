@@ -11,8 +11,12 @@ Schedules (plan.schedule()["mode"]): 0 one persistent launch with grid
 barriers between axes, 1 one persistent launch per axis, 2 per shape group a
 plane-kernel launch (x and y) and a line-pass launch (z); mode 2 exists when
 every transform has three axes of at least 16 points and x-y planes of at most
-128 x 128.  An untuned plan runs mode 2 where it exists, else mode 1; neither
-waits on a grid barrier.  Only check() reports a mode-0 barrier timeout.
+128 x 128.  3 one persistent dataflow launch of the mode-2 tiles (FP32 batches
+of 32^3, 64^3 and 128^3 cubes, 128^3 out of place): a transform's z tiles wait
+on a counter of its own plane tiles, no grid barrier, one work queue per XCD.
+An untuned plan runs mode 2 where it exists, else mode 1; neither waits on a
+grid barrier.  Only check() reports a timed-out wait (mode 0 barrier, mode 3
+dependency).
 
 Plans and graphs made before fpga_final (or a new fpga_initialize) are stale:
 calling them raises RuntimeError; create them again.
@@ -100,9 +104,9 @@ class MultiPlan:
                 "phases": ph.value}
 
     def set_schedule(self, mode: int, grid: int, chunk: int):
-        """mode 0, 1 or 2 (see the module doc); grid (resident workgroups) and
-        chunk (tiles per queue ticket; 0 = static round-robin, no tickets) apply to
-        the persistent modes."""
+        """mode 0, 1, 2 or 3 (see the module doc); grid (resident workgroups, 0
+        = all) and chunk (tiles per queue ticket; 0 = static round-robin, no
+        tickets, modes 0 and 1 only) apply to the persistent modes."""
         if self._lib.fftamd_multi_set_schedule(self._h, mode, grid, chunk) != 0:
             raise ValueError(last_error())
 

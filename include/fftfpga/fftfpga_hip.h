@@ -113,10 +113,14 @@ int fftfpgaf_c2c_3d_ddr_dev(unsigned N, const float2 *d_in, float2 *d_out, bool 
  * N per call (api/src/fft3d.c:396-913); this plan runs every pass of every
  * transform in a single persistent kernel: per phase (axis), 256-thread
  * workgroups take chunks of tiles from a device-side work queue, largest
- * transforms first, with a grid barrier between phases.  Schedules (single
- * launch, one launch per phase, or plane and z-pass kernels per shape group;
- * workgroups per CU, tiles per queue ticket) are picked by
- * fftamd_multi_tune and cached per shape signature. */
+ * transforms first, with a grid barrier between phases; or, for batches of
+ * 32^3, 64^3 and 128^3 cubes, one dataflow launch of both passes of every
+ * transform (plane tiles and z lines, or the y-split pair for 128^3) from one
+ * work queue per XCD, where a transform's second pass waits only for its own
+ * first pass.  Schedules (single launch, one launch per phase, plane and
+ * z-pass kernels per shape group, or the dataflow launch; workgroups per CU,
+ * tiles per queue ticket) are picked by fftamd_multi_tune and cached per shape
+ * signature. */
 typedef struct fftamd_multi_s *fftamd_multi_t;
 typedef struct fftamd_graph_s *fftamd_graph_t;
 
@@ -132,8 +136,9 @@ int fftamd_multi_create(fftamd_multi_t *plan, unsigned count, const fftamd_xform
                         const float2 *d_in, float2 *d_out, int inverse, unsigned flags);
 /* Enqueue the plan on a stream (asynchronous). */
 int fftamd_multi_exec(fftamd_multi_t plan, void *stream);
-/* Synchronise the stream and report whether the last launch's grid barrier
- * wait exceeded its bound (-1 with a message) or not (0). */
+/* Synchronise the stream and report whether a wait of the last launch (grid
+ * barrier in mode 0, dependency in mode 3) exceeded its bound (-1 with a
+ * message) or not (0). */
 int fftamd_multi_check(fftamd_multi_t plan, void *stream);
 /* Time every schedule candidate on the plan's buffers (reps launches each),
  * keep the fastest and append it to cache_path (NULL: no cache).  Returns 0
@@ -146,15 +151,23 @@ int fftamd_multi_tune(fftamd_multi_t plan, void *stream, const char *cache_path,
  * x and y passes as one plane-kernel launch (a whole plane per workgroup, in
  * LDS) and the z passes as one line-pass launch, both over a table of the
  * group's transforms (128^3 groups whose inputs and outputs do not overlap:
- * the y-split pair instead; grid and chunk do not apply).  chunk: tiles per
- * queue ticket in modes 0 and 1, or 0 for a static round-robin assignment of
- * tiles to workgroups (no tickets).  The tuner considers every available mode. */
+ * the y-split pair instead; grid and chunk do not apply); mode 3 (FP32 plans
+ * whose transforms are all 32^3, 64^3 or 128^3, 128^3 out of place): the
+ * same tiles in one persistent launch of 512-thread workgroups, ordered by a
+ * counter per transform instead of barriers, from one work queue per XCD (a
+ * transform's tiles all in one queue, its second pass after its first), so
+ * that it completes with any number of resident workgroups.  chunk: tiles
+ * per queue ticket in modes 0, 1 and 3, or 0 for a static round-robin
+ * assignment of tiles to workgroups (no tickets; modes 0 and 1).  grid:
+ * workgroups of the persistent kernel, at most those resident at once (0:
+ * all of them).  The tuner considers every available mode. */
 int fftamd_multi_schedule(fftamd_multi_t plan, int *mode, unsigned *grid, unsigned *chunk,
                           unsigned *items, unsigned *phases);
 int fftamd_multi_set_schedule(fftamd_multi_t plan, int mode, unsigned grid, unsigned chunk);
 void fftamd_multi_destroy(fftamd_multi_t plan);
-/* Bound on the grid barrier's polls (default 2^22); a tiny value forces the
- * timeout path so callers can test their error handling. */
+/* Bound on the polls of a grid barrier or dependency wait (default 2^22); a
+ * tiny value forces the timeout path so callers can test their error
+ * handling. */
 void fftamd_multi_set_spin_limit(unsigned polls);
 
 /* Capture plans[0..n-1] executed in order (e.g. a forward/inverse pair) into a
@@ -196,6 +209,15 @@ int fftamd_profile_end(char *json_out, int cap);
  * writes at most cap values and returns the pass count, or -1 on error. */
 int fftamd_describe_plan(int rank, const unsigned *dims, unsigned batch, unsigned flags, int prec,
                          int aliased, long long *out, int cap);
+
+/* Describe, without touching the device, the work queues of the dataflow
+ * schedule (mode 3) for `count` transforms of dims3[3 i .. 3 i + 2], each
+ * 32^3, 64^3 or 128^3: qstart[0..8] receives the queue starts (one queue per
+ * XCD) and, when out is not NULL, out[3 j .. 3 j + 2] the j-th tile as
+ * (transform index, pass 0 or 1, tile within that pass).  Returns the tile
+ * count, or -1 on error (a transform of another shape, cap too small). */
+int fftamd_describe_flow(unsigned count, const unsigned *dims3, unsigned *qstart, unsigned *out,
+                         unsigned cap);
 
 #ifdef __cplusplus
 }

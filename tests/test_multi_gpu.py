@@ -2,11 +2,11 @@
 
 Bars: every transform of a mixed batch within rel-L2 2e-6 of the FP64
 oracle; each schedule runs the same tile code as one per-transform path
-(modes 0 and 1: the line passes; mode 2: the plane kernel and the z line
-pass), so on the 448-transform workload its output must equal that path's
-output bit for bit; HIP-graph replays must equal the eager
-result on every replay; a grid-barrier wait forced past its bound from the
-host must be reported, not hang.
+(modes 0 and 1: the line passes; modes 2 and 3: the plane kernel and the z
+line pass, or the y-split pair for 128^3), so on the 448-transform workload
+its output must equal that path's output bit for bit; HIP-graph replays must
+equal the eager result on every replay; a grid-barrier or dependency wait
+forced past its bound from the host must be reported, not hang.
 """
 import sys
 
@@ -131,7 +131,7 @@ def _config3_shapes():
     return [(n, n, n) for n in sizes]
 
 
-@pytest.mark.parametrize("mode,chunk", [(0, None), (1, None), (2, None), (0, 0)])
+@pytest.mark.parametrize("mode,chunk", [(0, None), (1, None), (2, None), (0, 0), (3, 1), (3, 4)])
 def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode, chunk):
     import torch
     from fftfpga_amd import c2c_dev
@@ -142,15 +142,15 @@ def test_config3_448_uneven_matches_per_transform_bitwise(gpu_lib, orc, mode, ch
     d_out = torch.empty_like(d_in)
     plan = MultiPlan(shapes, d_in, d_out)
     sch = plan.schedule()
-    plan.set_schedule(mode, sch["grid"], sch["chunk"] if chunk is None else chunk)
+    plan.set_schedule(mode, 0 if mode == 3 else sch["grid"], sch["chunk"] if chunk is None else chunk)
     plan()
     plan.check()
     ref = torch.empty_like(d_in)
     for s, o in zip(shapes, offs):
         n = int(np.prod(s))
-        # mode 2 (out of place) runs the same kernels as a single call: plane kernels for
-        # 32^3 and 64^3, the y-split pair for 128^3
-        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s), plane_fusion=(mode == 2))
+        # modes 2 and 3 (out of place) run the same tiles as a single call: plane kernels
+        # for 32^3 and 64^3, the y-split pair for 128^3
+        c2c_dev(d_in[o:o + n].view(*s), 3, out=ref[o:o + n].view(*s), plane_fusion=(mode in (2, 3)))
     torch.cuda.synchronize()
     assert torch.equal(d_out, ref)
     out = d_out.cpu().numpy()
@@ -323,7 +323,7 @@ def test_autotuner_picks_and_caches(gpu_lib, tmp_path):
     cache = str(tmp_path / "tune.txt")
     plan = MultiPlan(shapes, d_in, d_out)
     r1 = plan.tune(cache, reps=2)
-    assert not r1["from_cache"] and r1["best_us"] > 0 and r1["mode"] in (0, 1, 2)
+    assert not r1["from_cache"] and r1["best_us"] > 0 and r1["mode"] in (0, 1, 2, 3)
     plan2 = MultiPlan(shapes, d_in, d_out)
     r2 = plan2.tune(cache, reps=2)
     assert r2["from_cache"] and (r2["mode"], r2["grid"], r2["chunk"]) == (r1["mode"], r1["grid"], r1["chunk"])
@@ -422,6 +422,123 @@ def test_untuned_plan_has_no_grid_barrier(gpu_lib):
     d_out = torch.empty_like(d_in)
     plan = MultiPlan(shapes, d_in, d_out)
     assert plan.schedule()["mode"] == 1
+    plan()
+    plan.check()
+    plan.close()
+
+
+# ---- mode 3: one dataflow launch (a counter per transform, one queue per XCD) ----
+CUBES = [(64, 64, 64), (32, 32, 32), (128, 128, 128), (32, 32, 32), (64, 64, 64), (128, 128, 128),
+         (32, 32, 32), (32, 32, 32), (64, 64, 64), (32, 32, 32), (128, 128, 128)]
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+@pytest.mark.parametrize("grid,chunk", [(0, 1), (0, 4), (3, 2)])
+def test_flow_schedule_vs_oracle(gpu_lib, orc, inverse, grid, chunk):
+    """Mode 3 against the oracle, with every resident workgroup and with three
+    (fewer than the eight queues: the launch must still drain every queue)."""
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    host, d_in, offs = _pack(CUBES, 11)
+    d_out = torch.zeros_like(d_in)
+    plan = MultiPlan(CUBES, d_in, d_out, inverse=inverse)
+    plan.set_schedule(3, grid, chunk)
+    plan()
+    plan.check()
+    out = d_out.cpu().numpy()
+    for s, o in zip(CUBES, offs):
+        n = int(np.prod(s))
+        assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s), inverse=inverse)) <= TOL, s
+    plan.close()
+
+
+def test_flow_schedule_in_place(gpu_lib, orc):
+    """32^3 and 64^3 transforms in place (their first pass runs plane by plane)."""
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    shapes = [s for s in CUBES if s[0] < 128]
+    host, d, offs = _pack(shapes, 12)
+    plan = MultiPlan(shapes, d, d)
+    plan.set_schedule(3, 0, 2)
+    plan()
+    plan.check()
+    out = d.cpu().numpy()
+    for s, o in zip(shapes, offs):
+        n = int(np.prod(s))
+        assert orc.rel_l2(out[o:o + n], orc.fftnd(host[o:o + n], list(s))) <= TOL, s
+    plan.close()
+
+
+def test_flow_schedule_unavailable_is_rejected(gpu_lib):
+    import torch
+    from fftfpga_amd.multi import MultiPlan
+    _, d_in, _ = _pack(PLANES, 3)  # not all cubes of 32, 64 or 128
+    plan = MultiPlan(PLANES, d_in, torch.empty_like(d_in))
+    with pytest.raises(ValueError):
+        plan.set_schedule(3, 0, 1)
+    plan.close()
+    shapes = [(128, 128, 128)]
+    _, d, _ = _pack(shapes, 3)
+    plan = MultiPlan(shapes, d, d)  # 128^3 in place: the pair's pass A needs another buffer
+    with pytest.raises(ValueError):
+        plan.set_schedule(3, 0, 1)
+    plan.close()
+
+
+def test_flow_graph_pair_replays(gpu_lib):
+    """A mode-3 forward/inverse pair captured in a HIP graph: every replay
+    equals the eager pair, with eager work between the replays."""
+    import torch
+    from fftfpga_amd.multi import Graph, MultiPlan
+    host, d_in, offs = _pack(CUBES, 13)
+    d_mid = torch.zeros_like(d_in)
+    d_back = torch.zeros_like(d_in)
+    fwd = MultiPlan(CUBES, d_in, d_mid)
+    inv = MultiPlan(CUBES, d_mid, d_back, inverse=True)
+    for p in (fwd, inv):
+        p.set_schedule(3, 0, 2)
+    fwd()
+    inv()
+    torch.cuda.synchronize()
+    eager_mid, eager_back = d_mid.clone(), d_back.clone()
+    g = Graph([fwd, inv])
+    for r in range(3):
+        d_mid.zero_()
+        d_back.mul_(0)
+        d_in.add_(0)  # eager work between replays
+        torch.cuda.synchronize()
+        g()
+        torch.cuda.synchronize()
+        assert torch.equal(d_mid, eager_mid) and torch.equal(d_back, eager_back), f"replay {r}"
+        fwd.check()
+        inv.check()
+    for s, o in zip(CUBES, offs):
+        n = int(np.prod(s))
+        back = d_back[o:o + n].cpu().numpy() / n
+        assert np.linalg.norm(back - host[o:o + n]) <= 2e-6 * np.linalg.norm(host[o:o + n])
+    g.close()
+    fwd.close()
+    inv.close()
+
+
+def test_flow_dependency_bound_is_reported_not_hung(gpu_lib):
+    """One 128^3 transform: its pass-B tiles are taken while its pass-A tiles
+    run, so with the poll bound forced to one from the host they give up,
+    report through check() and the launch still ends."""
+    import torch
+    from fftfpga_amd.multi import MultiPlan, set_spin_limit
+    shapes = [(128, 128, 128)]
+    _, d_in, _ = _pack(shapes, 3)
+    d_out = torch.empty_like(d_in)
+    plan = MultiPlan(shapes, d_in, d_out)
+    plan.set_schedule(3, 0, 1)
+    set_spin_limit(1)
+    try:
+        plan()
+        with pytest.raises(RuntimeError, match="bound"):
+            plan.check()
+    finally:
+        set_spin_limit(1 << 22)
     plan()
     plan.check()
     plan.close()
