@@ -52,7 +52,7 @@ constexpr int CA_LDS = CaRow::LDS_ELEMS > CaCol::LDS_ELEMS ? CaRow::LDS_ELEMS : 
 // Two workgroups per CU: at most 128 VGPRs (4 waves per SIMD), <= 74 KiB LDS each.
 template <bool INV, bool WL, bool SPLIT>
 __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
-  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(CA_LDS, CB_NT) : CA_LDS];
+  __shared__ cpx<float> buf[CA_LDS];
   __shared__ cpx<float> tws[16 * 15];  // row stage-1 twiddles W256^{k r} at [15 k + r - 1]
   __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   const unsigned long long plane = blockIdx.x >> 2;
@@ -85,6 +85,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
     }
   }
   __builtin_amdgcn_sched_barrier(0);  // keep the second half's loads ahead of the first half's arithmetic
+  if constexpr (SPLIT) __syncthreads();  // the DFT operand table is written
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     cpx<float>* v = h ? r1 : r0;
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
     // from the LDS table (lanes of one tau read one address)
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * tau + r - 1]);
-    wg_dft16<SPLIT, true>(v, buf, atab);  // the buffer has been read
+    wg_dft16<SPLIT, true>(v, atab);
   }
 
   // ---- y over b: columns kx = 128 hc + cc, 64-point lines; thread (cc, tq)
@@ -147,7 +148,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
     }
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[4 * tq * r]);
-    wg_dft16<SPLIT, true>(v, buf, atab);  // the buffer has been read
+    wg_dft16<SPLIT, true>(v, atab);
     // column DFT point k = tq + 4 r to row 64 res + k (pass B applies W256^{res k})
 #pragma unroll
     for (int r = 0; r < 16; ++r)
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_a(const CubeArgs a) {
 
 template <bool INV, bool WS, bool SPLIT>
 __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
-  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(CbZ::LDS_ELEMS, CB_NT) : CbZ::LDS_ELEMS];
+  __shared__ cpx<float> buf[CbZ::LDS_ELEMS];
   __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   // k varies fastest over blockIdx: the tiles in flight together then cover
   // all 64 rows k (measured on this access pattern alone, in place after pass
@@ -240,14 +241,14 @@ __global__ void __launch_bounds__(CB_NT, 4) k_fft_cube_b(const CubeArgs a) {
 
 template <bool INV, bool WL, bool SPLIT>
 __global__ void __launch_bounds__(C1_NT, 4) k_fft_cube128_a(const CubeArgs a) {
-  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C1A_LDS, C1_NT) : C1A_LDS];
+  __shared__ cpx<float> buf[C1A_LDS];
   __shared__ cpx<float> tws[C1A_TWS];
   cube128_a_tile<INV, WL, SPLIT>(a, blockIdx.x, buf, tws, (int)threadIdx.x);
 }
 
 template <bool INV, bool WS, bool SPLIT>
 __global__ void __launch_bounds__(C1B_NT, 4) k_fft_cube128_b(const CubeArgs a) {
-  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C1Z::LDS_ELEMS, C1B_NT) : C1Z::LDS_ELEMS];
+  __shared__ cpx<float> buf[C1Z::LDS_ELEMS];
   __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   if constexpr (SPLIT) split_atab_fill(atab);
   cube128_b_tile<INV, WS, SPLIT>(a, blockIdx.x, buf, atab, (int)threadIdx.x);

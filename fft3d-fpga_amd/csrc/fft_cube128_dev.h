@@ -31,7 +31,7 @@ constexpr int C1A_BUF = C1A_LDS;       // pass A tile's buffer (FP32), then 8 x 
 constexpr int C1A_TWS = 8 * 15;
 
 // Pass A tile `id` (plane id / 4, residue class id % 4) by threads tid < 256:
-// buf holds C1A_LDS elements (split: split_lds_cpx(C1A_LDS, 256)), tws 120.
+// buf holds C1A_LDS elements, tws 120.
 template <bool INV, bool WL, bool SPLIT>
 __device__ __forceinline__ void cube128_a_tile(const CubeArgs& a, unsigned long long id, cpx<float>* buf,
                                                cpx<float>* tws, int tid) {
@@ -78,7 +78,7 @@ __device__ __forceinline__ void cube128_a_tile(const CubeArgs& a, unsigned long 
   // the LDS table (lanes of one tau read one address)
 #pragma unroll
   for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], tws[15 * tau + r - 1]);
-  wg_dft16<SPLIT>(v, buf);  // the buffer has been read
+  wg_dft16<SPLIT>(v);
 
   // ---- y over b: 128 columns of 32 points; thread (cc, tq) ends with point
   // k = tq + 2 r of its column
@@ -97,7 +97,7 @@ __device__ __forceinline__ void cube128_a_tile(const CubeArgs& a, unsigned long 
   // radix-16 stage over sub-transforms of span 2: twiddles W32^{tq r} = W256^{8 tq r}
 #pragma unroll
   for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[8 * tq * r]);
-  wg_dft16<SPLIT>(v, buf);  // the buffer has been read
+  wg_dft16<SPLIT>(v);
   // column DFT point k = tq + 2 r to row 32 res + k (pass B applies W128^{res k})
 #pragma unroll
   for (int r = 0; r < 16; ++r) out[(long long)(32 * res + tq + 2 * r) * C1_L + cc] = v[r];
@@ -106,9 +106,9 @@ __device__ __forceinline__ void cube128_a_tile(const CubeArgs& a, unsigned long 
 // Pass B of 128^3 blocks: 32 adjacent columns (256-byte row segments, as the
 // 128-point z line pass; 16 columns measured 102.6 against 98.9 us) x 4 rows
 // x 128 planes (128 KiB) with 512 threads of 32 points, two workgroups per CU.
-// Pass B tile b by threads tid < 512: buf holds C1Z::LDS_ELEMS elements
-// (split: split_lds_cpx(C1Z::LDS_ELEMS, 512)); split: atab holds the DFT
-// operand table, filled before the tile's first barrier.
+// Pass B tile b by threads tid < 512: buf holds C1Z::LDS_ELEMS elements;
+// split: atab holds the DFT operand table, filled before the tile's first
+// barrier.
 template <bool INV, bool WS, bool SPLIT>
 __device__ __forceinline__ void cube128_b_tile(const CubeArgs& a, unsigned b, cpx<float>* buf,
                                                const split_half8* atab, int tid) {
@@ -169,7 +169,7 @@ __device__ __forceinline__ void cube128_b_tile(const CubeArgs& a, unsigned b, cp
     // W256^{2 tz r}, uniform over the wave (scalar loads)
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], wdir[2 * tz * r]);
-    wg_dft16<SPLIT, true>(v, buf, atab);  // the buffer has been read
+    wg_dft16<SPLIT, true>(v, atab);
     // point kz = tz + 8 r of row k + 32 d, column 32 xg + (cz & 31)
     const long long ob = (long long)(32 * (2 * e + (cz >> 5))) * C1_L + (cz & 31);
 #pragma unroll

@@ -57,7 +57,7 @@ constexpr int C5A_LDS = C5X_LDS > C5Col::LDS_ELEMS ? C5X_LDS : C5Col::LDS_ELEMS;
 // One workgroup per CU: 1024 threads (128 VGPRs each), <= 148 KiB of LDS.
 template <bool INV, bool WL, bool SPLIT>
 __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
-  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C5A_LDS, C5_NT) : C5A_LDS];
+  __shared__ cpx<float> buf[C5A_LDS];
   __shared__ cpx<float> w512[C5_L];  // W512^m
   __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   const unsigned long long plane = blockIdx.x >> 3;
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
     for (int t = 0; t < 16; ++t) y[t] = buf[(c * 16 + tau) * C5X_K + t];
 #pragma unroll
     for (int t = 1; t < 16; ++t) y[t] = cmul(y[t], w512[(k1 * t) & (C5_L - 1)]);
-    wg_dft16<SPLIT, true>(y, buf, atab);  // the buffer has been read
+    wg_dft16<SPLIT, true>(y, atab);
   }
 
   // ---- y over b: half R holds columns kx = 16 R + tau + 32 m (column index
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_a(const CubeArgs a) {
 
 template <bool INV, bool WS, bool SPLIT>
 __global__ void __launch_bounds__(C5_NT, 1) k_fft_c512_b(const CubeArgs a) {
-  __shared__ cpx<float> buf[SPLIT ? split_lds_cpx(C5Z::LDS_ELEMS, C5_NT) : C5Z::LDS_ELEMS];
+  __shared__ cpx<float> buf[C5Z::LDS_ELEMS];
   __shared__ cpx<float> w512[C5_L];
   __shared__ split_half8 atab[SPLIT ? SPLIT_ATAB : 1];  // split DFT operands (read after a barrier)
   // the two 8-column groups of one 128-byte row segment are blocks b and

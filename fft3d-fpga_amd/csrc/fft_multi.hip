@@ -66,9 +66,8 @@ __device__ __forceinline__ void dispatch_path(int path, const PassArgs& a, unsig
   }
 }
 
-// largest LDS footprint over the 16 paths (L <= 256, 256 threads), including
-// the per-wave scratch of split-precision stages
-constexpr int multi_lds_elems(bool split) {
+// largest LDS footprint over the 16 paths (L <= 256, 256 threads)
+constexpr int multi_lds_elems() {
   int m = 1;
   for (int lg = 1; lg <= MULTI_MAX_LG; ++lg) {
     const int L = 1 << lg;
@@ -77,8 +76,7 @@ constexpr int multi_lds_elems(bool split) {
     m = c0 > m ? c0 : m;
     m = c1 > m ? c1 : m;
   }
-  const int sc = (MULTI_THREADS / 64) * SPLIT_WAVE_FLOATS / 2;
-  return split && sc > m ? sc : m;
+  return m;
 }
 
 __global__ void k_multi_reset(MultiCtl* ctl) {
@@ -92,7 +90,7 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
     k_fft_multi(const MultiPass* __restrict__ passes, const MultiItem* __restrict__ items,
                 const unsigned* __restrict__ phase_start, unsigned phase0, unsigned nphase,
                 MultiCtl* ctl, unsigned chunk, unsigned spin_limit) {
-  __shared__ cpx<float> lds[multi_lds_elems(SPLIT)];
+  __shared__ cpx<float> lds[multi_lds_elems()];
   __shared__ unsigned s_ticket;
   __shared__ int s_abort;
   const unsigned nwg = gridDim.x;

@@ -24,8 +24,8 @@ __global__ void __launch_bounds__(geom_threads_for(L, MODE, SPLIT))
     k_fft_pass(const PassArgs a) {
   constexpr int CC = geom_cols_for(L, MODE, SPLIT);
   using G = Geom<L, MODE, CC>;
-  constexpr bool NEED_LDS = G::NP > 1 || SPLIT;
-  __shared__ cpx<R> lds[NEED_LDS ? (SPLIT ? G::LDS_ELEMS_SPLIT : G::LDS_ELEMS) : 1];
+  constexpr bool NEED_LDS = G::NP > 1;
+  __shared__ cpx<R> lds[NEED_LDS ? G::LDS_ELEMS : 1];
   pass_block<R, L, MODE, EPI, SPLIT, DIR, CC>(a, blockIdx.x, lds, (int)threadIdx.x);
 }
 

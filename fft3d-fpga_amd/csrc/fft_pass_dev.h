@@ -33,9 +33,6 @@ struct Geom {
   // contiguous: [line][p + p/16]; strided: [p][col] with one pad column
   static constexpr int LP = L + L / 16;
   static constexpr int LDS_ELEMS = MODE == MODE_CONTIG ? C * LP : L * (C + 1);
-  // split-precision radix-16 stages borrow the buffer as per-wave scratch
-  static constexpr int SPLIT_ELEMS = ((NT + 63) / 64) * SPLIT_WAVE_FLOATS / 2;
-  static constexpr int LDS_ELEMS_SPLIT = LDS_ELEMS > SPLIT_ELEMS ? LDS_ELEMS : SPLIT_ELEMS;
   static __device__ __forceinline__ int idx(int c, int p) {
     if constexpr (MODE == MODE_CONTIG) return c * LP + p + (p >> 4);
     else return p * (C + 1) + c;
@@ -83,11 +80,8 @@ __device__ __forceinline__ void stage_compute(cpx<R>* v, int tau, const cpx<R>* 
   }
   if constexpr (SPLIT && RAD == 16 && sizeof(R) == 4) {
     static_assert(NB == 1, "split radix-16 stages hold one butterfly per lane");
-    __syncthreads();  // every wave finished reading the stage buffer it is about to borrow
-    float* scratch = reinterpret_cast<float*>(lds) + (threadIdx.x >> 6) * SPLIT_WAVE_FLOATS;
-    if constexpr (TAB) dft16_split_mfma_tab(reinterpret_cast<cpx<float>*>(v), scratch, atab);
-    else dft16_split_mfma(reinterpret_cast<cpx<float>*>(v), scratch);
-    __syncthreads();  // scratch is released before the stage buffer is written again
+    (void)lds;
+    dft16_split_perm<TAB>(reinterpret_cast<cpx<float>*>(v), atab);
   } else {
 #pragma unroll
     for (int b = 0; b < NB; ++b) dft<R, RAD>(v + b * RAD);
@@ -135,8 +129,7 @@ __device__ __forceinline__ void run_stages(cpx<R>* v, cpx<R>* lds, int c, int ta
     }
     stage_compute<R, L, MODE, S, SPLIT, CC>(v, tau, tw, lds);
     if constexpr (S + 1 < G::NP) {
-      constexpr bool split_stage = SPLIT && sched_radix(L, S) == 16;
-      if constexpr (S > 0 && !split_stage) __syncthreads();  // every thread finished reading this stage
+      if constexpr (S > 0) __syncthreads();  // every thread finished reading this stage
       stage_store_lds<R, L, MODE, S, CC>(v, lds, c, tau);
       run_stages<R, L, MODE, S + 1, SPLIT, CC>(v, lds, c, tau, tw);
     }
@@ -322,9 +315,8 @@ __device__ __forceinline__ void tile_finish(const PassArgs& a, const TileLoc& t,
 // One workgroup tile of a pass (block index g), run by threads tid = 0 ..
 // Geom::NT - 1.  Used by k_fft_pass (one tile per workgroup) and by the
 // persistent multi-transform kernels (which may run two tiles side by side in
-// one workgroup, each half with its own tid range and LDS; split-precision
-// stages index their per-wave scratch by threadIdx.x and so need the whole
-// workgroup to be one tile).
+// one workgroup, each half with its own tid range and LDS, the halves made of
+// whole waves).
 template <typename R, int L, int MODE, int EPI, bool SPLIT = false, int DIR = -1,
           int CC = geom_cols(L, MODE)>
 __device__ __forceinline__ void pass_block(const PassArgs& a, unsigned long long g, cpx<R>* lds,

@@ -42,8 +42,7 @@ struct HalfGeom {
   using GY = Geom<L, MODE_STRIDED, L / 2>; // columns: L/2 lines of L points
   static constexpr int NT = L * L / 32;
   static_assert(GX::NT == NT && GY::NT == NT, "16 points per thread per half");
-  static constexpr int LDS0 = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
-  static constexpr int LDS = SPLIT && GX::SPLIT_ELEMS > LDS0 ? GX::SPLIT_ELEMS : LDS0;
+  static constexpr int LDS = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
   // last row stage: radix 16, one butterfly per thread, so point k = tau + TPL r
   static_assert(sched_radix(L, GX::NP - 1) == 16 && sched_ns(L, GX::NP - 1) == GX::TPL,
                 "points tau + TPL r: r < 8 are columns 0 .. L/2 - 1");

@@ -17,16 +17,13 @@ struct PlaneGeom {
   using GY = Geom<LY, MODE_STRIDED, LX>;   // columns: LX lines of LY points
   static constexpr int NT = LY * LX / 16;
   static_assert(GX::NT == NT && GY::NT == NT, "16 points per thread in both phases");
-  static constexpr int LDS0 = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
-  // split-precision stages borrow the buffer as per-wave operand scratch
-  static constexpr int LDS = SPLIT && GX::SPLIT_ELEMS > LDS0 ? GX::SPLIT_ELEMS : LDS0;
+  static constexpr int LDS = GX::LDS_ELEMS > GY::LDS_ELEMS ? GX::LDS_ELEMS : GY::LDS_ELEMS;
 };
 
 // One plane (index `plane` of the array, or of the transform table) by
 // threads tid = 0 .. NT - 1 with PlaneGeom::LDS elements of LDS at lds.
 // SPLIT: radix-16 stages on FP16 matrix cores with hi+lo operands
-// (fft_split.h; their per-wave scratch is indexed by threadIdx.x, so a split
-// tile must be the whole workgroup).
+// (fft_split.h; a tile is then made of whole waves).
 template <int LY, int LX, bool INV, bool SPLIT>
 __device__ __forceinline__ void plane_tile(const PlaneArgs& a, unsigned long long plane, cpx<float>* lds,
                                            int tid) {

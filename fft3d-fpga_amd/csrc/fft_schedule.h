@@ -60,10 +60,12 @@ constexpr int geom_cols(int L, int mode) {
                              : (L >= 1024 ? 16 : L >= 512 ? 32 : L >= 256 ? 16 : 256 / geom_tpl(L));
 }
 constexpr int geom_threads(int L, int mode) { return geom_cols(L, mode) * geom_tpl(L); }
-// Split-precision stages borrow LDS per wave: strided 512-point lines keep 16
-// columns there (32 measured slower with the larger operand scratch).
+// Columns per workgroup of a line pass in either precision mode (the
+// split-precision stages exchange their operands in registers and keep the
+// FP32 layout).
 constexpr int geom_cols_for(int L, int mode, bool split) {
-  return split && mode == MODE_STRIDED && L >= 512 ? 16 : geom_cols(L, mode);
+  (void)split;
+  return geom_cols(L, mode);
 }
 constexpr int geom_threads_for(int L, int mode, bool split) { return geom_cols_for(L, mode, split) * geom_tpl(L); }
 

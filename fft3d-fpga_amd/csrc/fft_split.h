@@ -19,9 +19,9 @@
 // for j = 0..7, and receives D[4(l/16)+e][l%16] for e = 0..3.
 //
 // Data moves between the "one vector per lane" form used by the radix stages
-// and the operand form through a per-wave LDS scratch of 64 rows x 36 floats
-// (32 values + 4 pad floats so the 16 lanes reading one operand slice hit
-// distinct banks).
+// and the operand form in registers: block transposes across the wave's four
+// 16-lane rows with gfx950's v_permlane32_swap / v_permlane16_swap, so the
+// stages need no LDS scratch and no barrier.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -31,9 +31,6 @@ namespace fftamd {
 
 typedef _Float16 split_half8 __attribute__((ext_vector_type(8)));
 typedef float split_float4 __attribute__((ext_vector_type(4)));
-
-constexpr int SPLIT_ROW = 36;                      // floats per scratch row
-constexpr int SPLIT_WAVE_FLOATS = 64 * SPLIT_ROW;  // per wave
 
 // cos(2 pi m / 16) for m mod 16, from the exact constants of the radix-16
 // butterfly (selects only: no indexed local array, which would live in scratch)
@@ -75,7 +72,8 @@ __device__ __forceinline__ void split_a_operands(int lane, split_half8& a1h, spl
 }
 
 // A workgroup's table of those operands in LDS, SPLIT_ATAB entries: lane l's
-// four at [4 l .. 4 l + 3].  Filled by the first wave; read after a barrier.
+// four at [4 l .. 4 l + 3].  Filled by the first wave; every reader must be
+// behind a workgroup barrier that follows the fill.
 constexpr int SPLIT_ATAB = 4 * 64;
 __device__ __forceinline__ void split_atab_fill(split_half8* atab) {
   if (threadIdx.x < 64) {
@@ -88,30 +86,78 @@ __device__ __forceinline__ void split_atab_fill(split_half8* atab) {
   }
 }
 
-// Radix-16 DFT (forward sign) of the 16 points v[0..15] held by every lane of
-// the wave, in place.  All 64 lanes must call it together; `scratch` is this
-// wave's private LDS region of SPLIT_WAVE_FLOATS floats.  TAB: the constant
-// operands are read from `atab` (an LDS table from split_atab_fill) at every
-// call, and every index of the call is one the compiler cannot see through, so
-// nothing of one call (operands, scratch addresses) stays live until the next;
+// ---- operands moved between lanes in registers ---------------------------------
+// A wave's four 16-lane rows exchange 4 x 4 blocks of registers with gfx950's
+// v_permlane32_swap (the upper two rows of one VGPR with the lower two of
+// another) and v_permlane16_swap (the odd rows of one with the even rows of
+// another): with groups G0..G3 of W VGPRs, the swaps on (G0, G2), (G1, G3) and
+// then (G0, G1), (G2, G3) leave in group g of row R what group R of row g held
+// (the lane layout was checked on the device with integer data).
+template <int W>
+__device__ __forceinline__ void split_xpose(unsigned (&x)[4 * W]) {
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    const auto a = __builtin_amdgcn_permlane32_swap(x[e], x[2 * W + e], false, false);
+    x[e] = a[0];
+    x[2 * W + e] = a[1];
+    const auto b = __builtin_amdgcn_permlane32_swap(x[W + e], x[3 * W + e], false, false);
+    x[W + e] = b[0];
+    x[3 * W + e] = b[1];
+  }
+#pragma unroll
+  for (int e = 0; e < W; ++e) {
+    const auto a = __builtin_amdgcn_permlane16_swap(x[e], x[W + e], false, false);
+    x[e] = a[0];
+    x[W + e] = a[1];
+    const auto b = __builtin_amdgcn_permlane16_swap(x[2 * W + e], x[3 * W + e], false, false);
+    x[2 * W + e] = b[0];
+    x[3 * W + e] = b[1];
+  }
+}
+
+typedef _Float16 split_half2 __attribute__((ext_vector_type(2)));
+typedef unsigned split_u4 __attribute__((ext_vector_type(4)));
+
+// Radix-16 DFT (forward sign) of the 16 points v[0..15] of every lane of the
+// wave, in place: each lane scales its own vector and splits it into hi and
+// lo FP16 halves, packed as the four 8-value B slices [re 0..7 | re 8..15 |
+// im 0..7 | im 8..15]; one block transpose hands lane 16 R + n the slice R of
+// vector 16 g + n for product g; the four products' results (4 re + 4 im
+// points of one vector per lane) go back through one more transpose and are
+// unscaled by the lane's own factor.  All 64 lanes must call it together (no
+// lane inactive).  TAB: the constant operands are read from `atab` (an LDS
+// table from split_atab_fill) at every call through a lane index the compiler
+// cannot see through, so nothing of one call stays live until the next;
 // without, they are formed in registers (and a kernel keeps them live across
 // its calls).
 template <bool TAB>
-__device__ __forceinline__ void dft16_split_mfma_impl(cpx<float>* v, float* scratch, const split_half8* atab) {
+__device__ __forceinline__ void dft16_split_perm(cpx<float>* v, const split_half8* atab) {
   int lane = threadIdx.x & 63;
   if constexpr (TAB) asm volatile("" : "+v"(lane));
-  const int row = lane & 15, part = lane >> 4;
-  // ---- this lane's vector -> scratch row `lane`
-  float* my = scratch + lane * SPLIT_ROW;
+  float mx = 0.0f;
 #pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    my[t] = v[t].x;
-    my[16 + t] = v[t].y;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // ---- constant operands, formed once this lane's points are out of registers
+  for (int t = 0; t < 16; ++t) mx = fmaxf(mx, fmaxf(fabsf(v[t].x), fabsf(v[t].y)));
+  int ex = 0;
+  (void)frexpf(mx, &ex);  // mx = f * 2^ex, f in [0.5, 1)
+  const int sh = mx > 0.0f ? 15 - ex : 0;
+  const float sc = ldexpf(1.0f, sh);
+  const float inv_s = ldexpf(1.0f, -sh);
+  unsigned hb[16], lb[16];  // slice j in words 4 j .. 4 j + 3 (values 2 d, 2 d + 1 in word 4 j + d)
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int t = 8 * (j & 1) + 2 * d;
+      const float x0 = (j < 2 ? v[t].x : v[t].y) * sc;
+      const float x1 = (j < 2 ? v[t + 1].x : v[t + 1].y) * sc;
+      _Float16 h0, l0, h1, l1;
+      split2(x0, &h0, &l0);
+      split2(x1, &h1, &l1);
+      hb[4 * j + d] = __builtin_bit_cast(unsigned, split_half2{h0, h1});
+      lb[4 * j + d] = __builtin_bit_cast(unsigned, split_half2{l0, l1});
+    }
+  split_xpose<4>(hb);
+  split_xpose<4>(lb);
   split_half8 a1h, a1l, a2h, a2l;
   if constexpr (TAB) {
     int li = lane;
@@ -123,91 +169,42 @@ __device__ __forceinline__ void dft16_split_mfma_impl(cpx<float>* v, float* scra
   } else {
     split_a_operands(lane, a1h, a1l, a2h, a2l);
   }
-
-  // Group g reads rows 16g..16g+15 and writes its results back into the same
-  // rows: the wave issues every read of the group before any of its writes,
-  // and one wave's LDS operations complete in order, so each group is done in
-  // place and only one group's values are live at a time.
+  unsigned o[32];  // product g: words 8 g .. 8 g + 3 real, 8 g + 4 .. 8 g + 7 imaginary
+  const split_float4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
-    // operand slice: vector (16g + row), values 8*part .. 8*part+7 of [re 0..15 | im 0..15]
-    float* rowp = scratch + (16 * g + row) * SPLIT_ROW;
-    float x[8];
-    float mx = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      x[j] = rowp[8 * part + j];
-      mx = fmaxf(mx, fabsf(x[j]));
-    }
-    // the vector's maximum over its four lanes (parts): each output column
-    // depends on its own vector only, so every vector takes its own scale
-    mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
-    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-    // power-of-two scale bringing the vector's maximum into [2^14, 2^15)
-    int ex = 0;
-    (void)frexpf(mx, &ex);  // mx = f * 2^ex, f in [0.5, 1)
-    const int sh = mx > 0.0f ? 15 - ex : 0;
-    const float s = ldexpf(1.0f, sh);
-    const float inv_s = ldexpf(1.0f, -sh);
-    split_half8 bh, bl;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      _Float16 h, l;
-      split2(x[j] * s, &h, &l);
-      bh[j] = h;
-      bl[j] = l;
-    }
-    const split_float4 z = {0.f, 0.f, 0.f, 0.f};
+    const split_half8 bh = __builtin_bit_cast(split_half8, split_u4{hb[4 * g], hb[4 * g + 1], hb[4 * g + 2], hb[4 * g + 3]});
+    const split_half8 bl = __builtin_bit_cast(split_half8, split_u4{lb[4 * g], lb[4 * g + 1], lb[4 * g + 2], lb[4 * g + 3]});
     split_float4 r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bh, z, 0, 0, 0);
     split_float4 q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bh, z, 0, 0, 0);
     r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bl, r, 0, 0, 0);
     q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2h, bl, q, 0, 0, 0);
     r = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, bh, r, 0, 0, 0);
     q = __builtin_amdgcn_mfma_f32_16x16x32_f16(a2l, bh, q, 0, 0, 0);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // this lane now holds points 4*part+e of vector 16g+row
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      rowp[4 * part + e] = r[e] * inv_s;
-      rowp[16 + 4 * part + e] = q[e] * inv_s;
+      // elements copied out first (a bit cast of a vector element expression
+      // read element 0 in this compiler, found with a device probe)
+      const float re = r[e], im = q[e];
+      o[8 * g + e] = __builtin_bit_cast(unsigned, re);
+      o[8 * g + 4 + e] = __builtin_bit_cast(unsigned, im);
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  split_xpose<8>(o);  // now words 8 j .. 8 j + 7: points 4 j .. 4 j + 3 of this lane's vector
 #pragma unroll
-  for (int t = 0; t < 16; ++t) v[t] = {my[t], my[16 + t]};
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      v[4 * j + e] = {__builtin_bit_cast(float, o[8 * j + e]) * inv_s, __builtin_bit_cast(float, o[8 * j + 4 + e]) * inv_s};
 }
 
-__device__ __forceinline__ void dft16_split_mfma(cpx<float>* v, float* scratch) {
-  dft16_split_mfma_impl<false>(v, scratch, nullptr);
-}
-__device__ __forceinline__ void dft16_split_mfma_tab(cpx<float>* v, float* scratch, const split_half8* atab) {
-  dft16_split_mfma_impl<true>(v, scratch, atab);
-}
-
-// Complex elements of a stage buffer of `base` elements that split-precision
-// variants (wg_dft16) also borrow as operand scratch for `nt` threads.
-constexpr int split_lds_cpx(int base, int nt) {
-  return base > (nt / 64) * SPLIT_WAVE_FLOATS / 2 ? base : (nt / 64) * SPLIT_WAVE_FLOATS / 2;
-}
-
-// Radix-16 DFT of v[0..15] in a workgroup kernel whose stage buffer `buf` is
-// free at this point (every thread has read what it needs from it) and holds
-// at least SPLIT_WAVE_FLOATS floats per wave: with SPLIT each wave borrows its
-// slice as operand scratch between two barriers, so every thread of the
-// workgroup must make the call (TAB: the constant operands come from the LDS
-// table `atab`); without, the FP32 butterfly.
+// Radix-16 DFT of v[0..15] in a workgroup kernel: with SPLIT on the matrix
+// cores (every thread of the wave must make the call; TAB: the constant
+// operands come from the LDS table `atab`), without, the FP32 butterfly.
 template <bool SPLIT, bool TAB = false>
-__device__ __forceinline__ void wg_dft16(cpx<float>* v, cpx<float>* buf, const split_half8* atab = nullptr) {
+__device__ __forceinline__ void wg_dft16(cpx<float>* v, const split_half8* atab = nullptr) {
   if constexpr (SPLIT) {
-    __syncthreads();  // every wave finished reading the buffer it is about to borrow
-    float* scratch = reinterpret_cast<float*>(buf) + (threadIdx.x >> 6) * SPLIT_WAVE_FLOATS;
-    if constexpr (TAB) dft16_split_mfma_tab(v, scratch, atab);
-    else dft16_split_mfma(v, scratch);
-    __syncthreads();  // scratch is released before the buffer is written again
+    dft16_split_perm<TAB>(v, atab);
   } else {
     dft16(v);
   }
