@@ -22,46 +22,52 @@
 namespace fftamd {
 
 template <int MODE, int LG, bool SPLIT>
-__device__ __forceinline__ void run_path_body(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
+__device__ __forceinline__ void run_path_body(const PassArgs& a, unsigned long long blk, cpx<float>* lds,
+                                              int tid) {
   static_assert(geom_threads(1 << LG, MODE) == MULTI_THREADS, "multi paths use 256-thread tiles");
   // split precision applies to lines with a radix-16 stage
-  pass_block<float, (1 << LG), MODE, EPI_NONE, SPLIT && (LG >= 4)>(a, blk, lds, (int)threadIdx.x);
+  pass_block<float, (1 << LG), MODE, EPI_NONE, SPLIT && (LG >= 4)>(a, blk, lds, tid);
 }
 
 // Split-precision paths run as one call per tile: out of line, each gets its
 // own register allocation (inlined together they exceed the 128-VGPR budget
 // of 4 waves per SIMD and spill heavily).  FP32 paths stay inlined.
 template <int MODE, int LG>
-__device__ __noinline__ void run_path_call(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
-  run_path_body<MODE, LG, true>(a, blk, lds);
+__device__ __noinline__ void run_path_call(const PassArgs& a, unsigned long long blk, cpx<float>* lds, int tid) {
+  run_path_body<MODE, LG, true>(a, blk, lds, tid);
 }
 
 template <int MODE, int LG, bool SPLIT>
-__device__ __forceinline__ void run_path(const PassArgs& a, unsigned long long blk, cpx<float>* lds) {
-  if constexpr (SPLIT) run_path_call<MODE, LG>(a, blk, lds);
-  else run_path_body<MODE, LG, false>(a, blk, lds);
+__device__ __forceinline__ void run_path(const PassArgs& a, unsigned long long blk, cpx<float>* lds, int tid) {
+  if constexpr (SPLIT) run_path_call<MODE, LG>(a, blk, lds, tid);
+  else run_path_body<MODE, LG, false>(a, blk, lds, tid);
 }
 
 template <bool SPLIT>
 __device__ __forceinline__ void dispatch_path(int path, const PassArgs& a, unsigned long long blk,
                                               cpx<float>* lds) {
+  // the thread index made opaque per tile, so that no address a tile derives
+  // from it is hoisted out of the persistent loop (and held in registers
+  // across every tile)
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   switch (path) {
-    case 0: run_path<MODE_CONTIG, 1, SPLIT>(a, blk, lds); break;
-    case 1: run_path<MODE_CONTIG, 2, SPLIT>(a, blk, lds); break;
-    case 2: run_path<MODE_CONTIG, 3, SPLIT>(a, blk, lds); break;
-    case 3: run_path<MODE_CONTIG, 4, SPLIT>(a, blk, lds); break;
-    case 4: run_path<MODE_CONTIG, 5, SPLIT>(a, blk, lds); break;
-    case 5: run_path<MODE_CONTIG, 6, SPLIT>(a, blk, lds); break;
-    case 6: run_path<MODE_CONTIG, 7, SPLIT>(a, blk, lds); break;
-    case 7: run_path<MODE_CONTIG, 8, SPLIT>(a, blk, lds); break;
-    case 8: run_path<MODE_STRIDED, 1, SPLIT>(a, blk, lds); break;
-    case 9: run_path<MODE_STRIDED, 2, SPLIT>(a, blk, lds); break;
-    case 10: run_path<MODE_STRIDED, 3, SPLIT>(a, blk, lds); break;
-    case 11: run_path<MODE_STRIDED, 4, SPLIT>(a, blk, lds); break;
-    case 12: run_path<MODE_STRIDED, 5, SPLIT>(a, blk, lds); break;
-    case 13: run_path<MODE_STRIDED, 6, SPLIT>(a, blk, lds); break;
-    case 14: run_path<MODE_STRIDED, 7, SPLIT>(a, blk, lds); break;
-    case 15: run_path<MODE_STRIDED, 8, SPLIT>(a, blk, lds); break;
+    case 0: run_path<MODE_CONTIG, 1, SPLIT>(a, blk, lds, tid); break;
+    case 1: run_path<MODE_CONTIG, 2, SPLIT>(a, blk, lds, tid); break;
+    case 2: run_path<MODE_CONTIG, 3, SPLIT>(a, blk, lds, tid); break;
+    case 3: run_path<MODE_CONTIG, 4, SPLIT>(a, blk, lds, tid); break;
+    case 4: run_path<MODE_CONTIG, 5, SPLIT>(a, blk, lds, tid); break;
+    case 5: run_path<MODE_CONTIG, 6, SPLIT>(a, blk, lds, tid); break;
+    case 6: run_path<MODE_CONTIG, 7, SPLIT>(a, blk, lds, tid); break;
+    case 7: run_path<MODE_CONTIG, 8, SPLIT>(a, blk, lds, tid); break;
+    case 8: run_path<MODE_STRIDED, 1, SPLIT>(a, blk, lds, tid); break;
+    case 9: run_path<MODE_STRIDED, 2, SPLIT>(a, blk, lds, tid); break;
+    case 10: run_path<MODE_STRIDED, 3, SPLIT>(a, blk, lds, tid); break;
+    case 11: run_path<MODE_STRIDED, 4, SPLIT>(a, blk, lds, tid); break;
+    case 12: run_path<MODE_STRIDED, 5, SPLIT>(a, blk, lds, tid); break;
+    case 13: run_path<MODE_STRIDED, 6, SPLIT>(a, blk, lds, tid); break;
+    case 14: run_path<MODE_STRIDED, 7, SPLIT>(a, blk, lds, tid); break;
+    case 15: run_path<MODE_STRIDED, 8, SPLIT>(a, blk, lds, tid); break;
     default: break;
   }
 }
