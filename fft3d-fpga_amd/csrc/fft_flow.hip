@@ -87,19 +87,27 @@ __device__ __forceinline__ void flow_tile(const FlowArgs& fa, const FlowItem& it
 // by thread 0 until the workgroup publishes them.
 constexpr int FLOW_PEND = 64;
 
-// Publish: every wave's stores reach memory (agent-scope release), then
-// thread 0 adds the pending tiles to their transforms' counters.
+// Publish: every wave's stores are ordered before a workgroup barrier
+// (workgroup-scope release), after which the first wave alone makes them
+// visible to the agent (one agent-scope release, which writes back the L2 that
+// all of the workgroup's stores have reached) and adds the pending tiles to
+// their transforms' counters.  One L2 writeback per publication instead of one
+// per wave.
 __device__ __forceinline__ void flow_publish(unsigned* done, const unsigned* pdep, const unsigned* pcnt,
                                              unsigned npend, int tid) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
-  if (tid == 0)
-    for (unsigned i = 0; i < npend; ++i)
-      __hip_atomic_fetch_add(done + (size_t)pdep[i] * FLOW_DONE_STRIDE, pcnt[i], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+  if (tid < 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (tid == 0)
+      for (unsigned i = 0; i < npend; ++i)
+        __hip_atomic_fetch_add(done + (size_t)pdep[i] * FLOW_DONE_STRIDE, pcnt[i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
-// An agent-scope release writes back the XCD's whole L2, so a workgroup
+// An agent-scope release writes back the XCD's whole L2 (and an acquire
+// invalidates it), so each is taken by one wave per workgroup, and a workgroup
 // publishes its finished first-pass tiles only before it waits (its first
 // second-pass tile after first-pass ones), before it leaves a queue, or when
 // FLOW_PEND runs are pending -- never while holding unpublished tiles in a
@@ -164,9 +172,12 @@ __global__ void __launch_bounds__(FLOW_THREADS, 4)
               }
               s_abort = abort;
             }
+            // the first wave takes the agent-scope acquire after the polls (it
+            // invalidates the CU's L1 and the L2 for the whole workgroup); the
+            // barrier then orders the other waves' loads after it
+            if (tid < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __syncthreads();
             if (s_abort) return;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             acquired = true;
           } else {
             __syncthreads();  // the previous tile's LDS reads are done

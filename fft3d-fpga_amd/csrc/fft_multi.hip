@@ -7,11 +7,12 @@
 // from a device-side work queue (one atomic ticket per chunk; tiles are
 // ordered largest transform first), run them with the same pass_block() as
 // the one-pass kernels, and meet at a grid barrier before the next phase:
-// every wave publishes its stores (agent-scope release), one thread per
-// workgroup arrives on a counter and waits for all workgroups (agent-scope
-// acquire).  The wait is bounded: past spin_limit polls the workgroup sets
-// ctl->err and exits, so a launch that cannot make progress ends with an
-// error instead of hanging.  The host sizes the grid to what is resident.
+// the workgroup publishes its stores (one agent-scope release by its first
+// wave), one thread per workgroup arrives on a counter and waits for all
+// workgroups (agent-scope acquire by the first wave).  The wait is bounded:
+// past spin_limit polls the workgroup sets ctl->err and exits, so a launch
+// that cannot make progress ends with an error instead of hanging.  The host
+// sizes the grid to what is resident.
 #include <hip/hip_runtime.h>
 
 #include "launch.h"
@@ -132,9 +133,13 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
       }
     }
     if (ph + 1 < nphase) {
-      // grid barrier between phases
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      // grid barrier between phases: the waves' stores are ordered before a
+      // workgroup barrier, then the first wave alone takes the agent-scope
+      // release and, after the wait, the acquire (on this chip each writes
+      // back or invalidates the L2, so one per workgroup instead of one per wave)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __syncthreads();
+      if (threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       if (threadIdx.x == 0) {
         // two levels: a workgroup arrives on its group's counter; the last of
         // its group in this phase arrives on the grid counter, which every
@@ -158,9 +163,9 @@ __global__ void __launch_bounds__(MULTI_THREADS, 4)
         }
         s_abort = abort;
       }
+      if (threadIdx.x < 64) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       __syncthreads();
       if (s_abort) return;
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
   }
 }
