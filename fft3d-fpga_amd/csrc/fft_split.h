@@ -150,11 +150,16 @@ __device__ __forceinline__ void dft16_split_perm(cpx<float>* v, const split_half
       const int t = 8 * (j & 1) + 2 * d;
       const float x0 = (j < 2 ? v[t].x : v[t].y) * sc;
       const float x1 = (j < 2 ? v[t + 1].x : v[t + 1].y) * sc;
-      _Float16 h0, l0, h1, l1;
-      split2(x0, &h0, &l0);
-      split2(x1, &h1, &l1);
-      hb[4 * j + d] = __builtin_bit_cast(unsigned, split_half2{h0, h1});
-      lb[4 * j + d] = __builtin_bit_cast(unsigned, split_half2{l0, l1});
+      // hi = fp16(x) for both values in one packed convert; the residuals
+      // x - hi are formed from that packed word (made opaque, so that the
+      // compiler does not convert each value a second time to get hi back)
+      unsigned hp = __builtin_bit_cast(unsigned, split_half2{(_Float16)x0, (_Float16)x1});
+      asm volatile("" : "+v"(hp));
+      const split_half2 h = __builtin_bit_cast(split_half2, hp);
+      const float r0 = x0 - (float)h[0];
+      const float r1 = x1 - (float)h[1];
+      hb[4 * j + d] = hp;
+      lb[4 * j + d] = __builtin_bit_cast(unsigned, split_half2{(_Float16)r0, (_Float16)r1});
     }
   split_xpose<4>(hb);
   split_xpose<4>(lb);
