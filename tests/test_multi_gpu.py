@@ -483,6 +483,12 @@ def test_flow_schedule_unavailable_is_rejected(gpu_lib):
     with pytest.raises(ValueError):
         plan.set_schedule(3, 0, 1)
     plan.close()
+    _, d_in, _ = _pack(CUBES, 3)
+    plan = MultiPlan(CUBES, d_in, torch.empty_like(d_in), split=True)  # FP32 tiles only
+    with pytest.raises(ValueError):
+        plan.set_schedule(3, 0, 1)
+    plan.set_schedule(2, 0, 1)  # the grouped launches take split plans
+    plan.close()
 
 
 def test_flow_graph_pair_replays(gpu_lib):
